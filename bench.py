@@ -1,0 +1,183 @@
+"""Benchmark of the fused D2Q9 stream-collide kernel on one MI355X.
+
+Workload (the line's ``value``): the 2048 x 2048 lid-driven cavity in FP64,
+Re = 1000, U = 0.1 (omega = 1/(3 U N / Re + 1/2) = 0.897343), started from
+rho = 1, u = 0 at f_eq.  A step is one fused pull stream + walls + collide
+pass over the whole grid (one kernel launch).  Populations stay in HBM; the
+timed region is K steps bracketed by device synchronisation, timed with HIP
+events on the stream the kernels are launched on.
+
+Also reported (``extra``): the same kernel on an 8192 x 8192 cavity (past the
+256 MB last-level cache: the size where an HBM-roofline fraction means
+something) and a device-to-device copy of the same byte count for reference.
+
+``roofline``: algorithmic bytes per lattice update = 2 * 9 * 8 (read and
+write all nine FP64 populations once) = 144 B; achieved = 144 * nodes /
+average launch duration; peak = 8 TB/s (MI355X HBM3E).  ``traffic`` is the
+HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters when a
+counter summary is committed under profiles/ (null otherwise).
+
+``cpu_baseline``: the oracle's numpy opt1 restatement of the reference path
+timed on one host core on a bounded sample of the same 2048^2 cavity.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LUP = {"f64": 144, "f32": 72, "bf16": 36, "f16": 36}
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000, help="timed steps (default: the "
+                    "20000 steps of the 2048^2 cavity workload)")
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
+    ap.add_argument("--cpu-budget", type=float, default=8.0)
+    return ap.parse_args()
+
+
+def timed_steps(lat, steps, warmup):
+    import torch
+    lat.step(warmup)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    lat.step(steps)
+    end.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / 1e3  # seconds
+
+
+def copy_bandwidth(nbytes):
+    """GB/s of a device-to-device copy moving ``nbytes`` read + ``nbytes`` written."""
+    import torch
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    start.record()
+    for _ in range(reps):
+        b.copy_(a)
+    end.record()
+    torch.cuda.synchronize()
+    sec = start.elapsed_time(end) / 1e3 / reps
+    del a, b
+    return 2 * nbytes / sec / 1e9
+
+
+def profiled_traffic(name):
+    """HBM bytes per launch from a committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(n, budget):
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
+           "--budget", str(budget), "--max-steps", "40"]
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600).stdout
+    rec = json.loads(out.strip().splitlines()[-1])
+    return {
+        "value": round(rec["numpy_opt1"]["mlups"], 4),
+        "unit": "MLUPS",
+        "cores": rec["cores"],
+        "kind": "port",
+        "sample": "numpy opt1 restatement of the %dx%d FP64 cavity step, %d steps (%.1f s) on "
+                  "one core of %s (%d visible); C restatement -O2: %.3f MLUPS"
+                  % (n, n, rec["numpy_opt1"]["steps"], rec["numpy_opt1"]["seconds"],
+                     rec["cpu_model"], rec["cpu_count"], rec["c_port"]["mlups"]),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the domain of every workload fits one MI355X); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from latticeboltzmann_amd import _native
+    from latticeboltzmann_amd.lattice import Lattice, omega_for_cavity
+
+    _native.require_gpu()
+    _native.load()
+    n = args.n
+    omega = omega_for_cavity(n, 1000.0, 0.1)
+    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant)
+    sec = timed_steps(lat, args.steps, args.warmup)
+    mlups = n * n * args.steps / sec / 1e6
+    launch_s = sec / args.steps
+    achieved = BYTES_PER_LUP["f64"] * n * n / launch_s / 1e9
+    ok = bool(torch.isfinite(lat.f).all())
+    del lat
+    torch.cuda.empty_cache()
+
+    extra = {"finite_state": ok, "omega": omega}
+    if not args.no_extra:
+        big = 8192
+        lat = Lattice(big, big, omega_for_cavity(big, 1000.0, 0.1), "cavity", 0.1,
+                      torch.float64, variant=args.variant)
+        steps = 100
+        s = timed_steps(lat, steps, 10)
+        m = big * big * steps / s / 1e6
+        extra["cavity_8192_f64"] = {
+            "mlups": round(m, 1), "steps": steps,
+            "achieved_gbs": round(BYTES_PER_LUP["f64"] * m * 1e6 / 1e9, 1),
+            "frac_of_8tbs": round(BYTES_PER_LUP["f64"] * m * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
+        }
+        del lat
+        torch.cuda.empty_cache()
+        nbytes = 9 * big * big * 8
+        extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
+
+    line = {
+        "metric": "MLUPS + % HBM roofline, FP64 vs FP16-store; time-to-steady-state cavity",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(launch_s * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (cavity from rest)",
+        "config": {"workload": "lid-driven cavity %dx%d FP64 D2Q9 BGK, Re=1000, U=0.1, "
+                               "omega=%.6f, fused pull stream-collide" % (n, n, omega),
+                   "nx": n, "ny": n, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("step_f64_cavity_%d_traffic.json" % n)},
+        "cpu_baseline": None,
+        "extra": extra,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(n, args.cpu_budget)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

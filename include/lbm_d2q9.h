@@ -1,0 +1,127 @@
+/*
+ * lbm_d2q9.h -- C ABI of the MI355X (gfx950) D2Q9 lattice-Boltzmann kernels.
+ *
+ * Every entry point takes plain device pointers, sizes and a hipStream_t passed
+ * as `void *` (NULL = the default stream), enqueues its kernels on that stream
+ * and returns without synchronising.  The return value is 0 on success, a
+ * negative LBM_E* code for an argument error (nothing is launched), or a
+ * positive hipError_t if a launch failed.
+ *
+ * Population arrays use the reference's layout f[i][k][l]: channel i = 0..8
+ * (0 rest, 1 E, 2 N, 3 W, 4 S, 5 NE, 6 NW, 7 SW, 8 SE; PyLB/Streaming.py:28-29),
+ * x index k = 0..nx-1 (slow), y index l = 0..ny-1 (fast, contiguous); a plane
+ * holds nx*ny values and the nine planes follow each other.  Scalar fields
+ * (rho, ux, uy) are [k][l].
+ *
+ * Arithmetic of the *_f64 / *_f32 entry points follows the reference's numpy
+ * opt1 operation order (cavity_opt1.py:100-245) without fused multiply-adds,
+ * so FP64 results are bit-identical to that path.
+ */
+#ifndef LBM_D2Q9_H
+#define LBM_D2Q9_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define LBM_OK 0
+#define LBM_E_NULL (-1)      /* a required pointer is NULL                    */
+#define LBM_E_SIZE (-2)      /* a size is zero, negative or too large         */
+#define LBM_E_ARG (-3)       /* an enum or scalar argument is out of range    */
+#define LBM_E_ALIAS (-4)     /* input and output buffers overlap              */
+#define LBM_E_WORKSPACE (-5) /* workspace too small                           */
+
+/* boundary kinds for the fused step */
+#define LBM_BC_PERIODIC 0 /* periodic in x and y (shear wave)                 */
+#define LBM_BC_CAVITY 1   /* bounce-back walls, moving lid at l = ny-1        */
+
+/* Library version string and the number of kernel variants of the fused step. */
+const char *lbm_version(void);
+int lbm_step_variant_count(void);
+
+/* ---- equilibrium ------------------------------------------------------------
+ * f[i][j] = f_eq_i(rho[j], ux[j], uy[j]) for j < n (f is 9 x n).
+ * Replaces _lbkernels.equilibrium, array overload -> d2q9_equilibriumn<T>
+ * (c/d2q9.h:98-108, bound at c/_lbkernels.cpp:38-39, 45-46), with the
+ * arithmetic of cavity_opt1.py:100-132.  The scalar overload
+ * d2q9_equilibrium1<T> (c/d2q9.h:59-81) is this call with n = 1.            */
+int lbm_equilibrium_f64(const double *rho, const double *ux, const double *uy,
+                        double *f, int64_t n, void *stream);
+int lbm_equilibrium_f32(const float *rho, const float *ux, const float *uy,
+                        float *f, int64_t n, void *stream);
+
+/* ---- collide ------------------------------------------------------------------
+ * BGK relaxation in place on f (9 x n): f += omega (f_eq(rho, u) - f).
+ * rho/ux/uy (each n values, all NULL or all non-NULL) receive the
+ * pre-collision moments, as the numpy collide returns them (cavity_opt1.py:160).
+ * Replaces _lbkernels.collide -> d2q9_colliden<T> (c/d2q9.h:121-131, bound at
+ * c/_lbkernels.cpp:40-41, 47-48); arithmetic of cavity_opt1.py:134-160.      */
+int lbm_collide_f64(double *f, int64_t n, double omega, double *rho, double *ux,
+                    double *uy, void *stream);
+int lbm_collide_f32(float *f, int64_t n, float omega, float *rho, float *ux,
+                    float *uy, void *stream);
+
+/* Moments of f (9 x n) without modifying it, in the order of the numpy
+ * collide (cavity_opt1.py:156-158); the reference computes them inline in its
+ * drivers before each dump (cavity_opt2.py:280-281).                          */
+int lbm_moments_f64(const double *f, int64_t n, double *rho, double *ux, double *uy,
+                    void *stream);
+int lbm_moments_f32(const float *f, int64_t n, float *rho, float *ux, float *uy, void *stream);
+
+/* ---- streaming ------------------------------------------------------------------
+ * Periodic pull streaming src -> dst (9 x nx x ny each, no overlap):
+ * dst_i[k][l] = src_i[(k - cx_i) mod nx][(l - cy_i) mod ny].  A pure
+ * permutation of elements of elem_bytes in {1, 2, 4, 8}.
+ * Replaces PyLB.stream (PyLB/Streaming.py:33-46).                              */
+int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
+                        int elem_bytes, void *stream);
+
+/* Streaming plus cavity walls: bounce-back at l = 0, k = 0, k = nx-1 and the
+ * lid at l = ny-1 moving with u0 in +x, with the reference's override order.
+ * Replaces stream_and_bounce_back (cavity_opt1.py:177-245, identical at
+ * cavity_opt2.py:109-177).                                                      */
+int lbm_stream_bounce_back_f64(const double *src, double *dst, int64_t nx,
+                               int64_t ny, double u0, void *stream);
+int lbm_stream_bounce_back_f32(const float *src, float *dst, int64_t nx,
+                               int64_t ny, float u0, void *stream);
+
+/* ---- fused step -------------------------------------------------------------------
+ * One full time step src -> dst: pull streaming (+ walls and lid if
+ * bc == LBM_BC_CAVITY), then BGK collide, reading and writing each population
+ * once.  Equivalent to the driver loops cavity_opt1.py:341-343
+ * (stream_and_bounce_back; collide) and shear_wave_opt1.py:171-172
+ * (stream; collide).  rho/ux/uy (nullable, all or none) receive the
+ * pre-collision moments.  variant selects a launch shape
+ * (0 .. lbm_step_variant_count()-1, or -1 for the default); all variants give
+ * identical results.                                                            */
+int lbm_step_f64(const double *src, double *dst, int64_t nx, int64_t ny,
+                 double omega, int bc, double u0, double *rho, double *ux,
+                 double *uy, int variant, void *stream);
+int lbm_step_f32(const float *src, float *dst, int64_t nx, int64_t ny,
+                 float omega, int bc, float u0, float *rho, float *ux, float *uy,
+                 int variant, void *stream);
+
+/* ---- reductions (FP64, deterministic) -------------------------------------------
+ * Two-pass reductions with a fixed block count: identical inputs give
+ * bit-identical results.  `work` is device scratch of at least
+ * lbm_reduce_workspace_bytes() bytes; *out is a device double.
+ * No reference counterpart: the reference prints no mass or residual.          */
+int64_t lbm_reduce_workspace_bytes(void);
+/* sum of n doubles (mass of an FP64 population array: n = 9 nx ny) */
+int lbm_sum_f64(const double *x, int64_t n, double *work, double *out, void *stream);
+/* sum of n floats accumulated in FP64 */
+int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stream);
+/* Velocity residual: out = max_j |(ux, uy)(f)_j - (ux_prev, uy_prev)_j| over
+ * the nx*ny nodes of an FP64 population array f; ux_prev/uy_prev are then
+ * overwritten with the current velocity.                                       */
+int lbm_velocity_residual_f64(const double *f, int64_t n, double *ux_prev,
+                              double *uy_prev, double *work, double *out,
+                              void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LBM_D2Q9_H */

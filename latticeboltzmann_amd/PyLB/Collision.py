@@ -1,0 +1,132 @@
+"""``PyLB.equilibrium`` and ``PyLB.collide`` on the GPU.
+
+Mirrors the reference's pybind11 module ``_lbkernels`` (``c/_lbkernels.cpp:33-49``)
+as re-exported by ``PyLB/Collision.py:21``:
+
+``equilibrium(rho, ux, uy) -> ndarray (9,)``
+    scalar overload (``d2q9_equilibrium1``, ``c/d2q9.h:59-81``).  pybind11
+    registers the float overloads first (``c/_lbkernels.cpp:36-41``) and its
+    float caster accepts any Python number, so the reference always answers
+    in float32; so does this one.
+``equilibrium(rho_kl, ux_kl, uy_kl, f_ikl) -> None``
+    array overload (``d2q9_equilibriumn``, ``c/d2q9.h:98-108``): writes f_eq
+    into ``f_ikl`` of shape ``(9, N)`` in place.
+``collide(f_ikl, omega) -> None``
+    BGK collide in place on ``f_ikl`` of shape ``(9, N)``
+    (``d2q9_colliden``, ``c/d2q9.h:121-131``).
+
+Argument rules follow the reference's ``Eigen::Ref`` bindings: arrays must
+be float32 or float64, all of one dtype, writable, and of the stated rank;
+otherwise ``TypeError``.  Stricter than the reference: arrays must be
+C-contiguous (the reference silently mis-indexes row-strided views,
+``c/d2q9.h:105,125``) and lengths must agree (unchecked there); violations
+raise ``TypeError`` and ``ValueError`` respectively.
+
+The arithmetic is that of the reference's numpy opt1 path
+(``cavity_opt1.py:100-160``), which agrees with ``_lbkernels`` to the 1e-7
+tolerance of ``tests/02-CollideTest.py``.
+"""
+
+import numbers
+
+import numpy as np
+import torch
+
+from .. import kernels
+from . import _staging
+
+_FLOAT_TYPES = (np.dtype(np.float32), np.dtype(np.float64))
+_TORCH_FLOATS = (torch.float32, torch.float64)
+
+
+def _incompatible(name):
+    return TypeError("%s(): incompatible function arguments" % name)
+
+
+def _dtype_of(a):
+    if isinstance(a, np.ndarray):
+        return a.dtype
+    if _staging.is_device_tensor(a):
+        return a.dtype
+    return None
+
+
+def _check_array(a, name, ndim):
+    """Reject what the reference's Eigen::Ref binding rejects (TypeError)."""
+    if isinstance(a, np.ndarray):
+        if a.dtype not in _FLOAT_TYPES or not a.flags.writeable:
+            raise _incompatible(name)
+        if not a.flags.c_contiguous:
+            raise TypeError("%s(): arrays must be C-contiguous" % name)
+    elif _staging.is_device_tensor(a):
+        if a.dtype not in _TORCH_FLOATS:
+            raise _incompatible(name)
+        if not a.is_contiguous():
+            raise TypeError("%s(): tensors must be contiguous" % name)
+    else:
+        raise _incompatible(name)
+    shape = tuple(a.shape)
+    if ndim == 1 and not (len(shape) == 1 or (len(shape) == 2 and shape[1] == 1)):
+        raise _incompatible(name)
+    if ndim == 2 and (len(shape) != 2 or shape[0] != 9):
+        raise _incompatible(name)
+
+
+def _device(a):
+    return a if _staging.is_device_tensor(a) else _staging.to_device(a)
+
+
+def _scalar_equilibrium(rho, ux, uy):
+    for x in (rho, ux, uy):
+        if isinstance(x, (bool, np.bool_)) or not isinstance(x, (numbers.Real, np.floating, np.integer)):
+            if not (isinstance(x, np.ndarray) and x.ndim == 0):
+                raise _incompatible("equilibrium")
+    dev = _staging.device()
+    vals = torch.tensor([float(rho), float(ux), float(uy)], dtype=torch.float32, device=dev)
+    out = torch.empty(9, dtype=torch.float32, device=dev)
+    kernels.equilibrium(vals[0:1], vals[1:2], vals[2:3], out=out)
+    return out.cpu().numpy()
+
+
+def equilibrium(*args):
+    """Equilibrium distribution; see the module docstring for both overloads."""
+    if len(args) == 3:
+        return _scalar_equilibrium(*args)
+    if len(args) != 4:
+        raise _incompatible("equilibrium")
+    rho, ux, uy, f = args
+    for a in (rho, ux, uy):
+        _check_array(a, "equilibrium", 1)
+    _check_array(f, "equilibrium", 2)
+    dtypes = {str(_dtype_of(a)).replace("torch.", "") for a in args}
+    if len(dtypes) != 1:
+        raise _incompatible("equilibrium")
+    kinds = {isinstance(a, np.ndarray) for a in args}
+    if len(kinds) != 1:
+        raise TypeError("equilibrium(): mix of numpy arrays and device tensors")
+    n = f.shape[1]
+    if any(a.shape[0] != n for a in (rho, ux, uy)):
+        raise ValueError("equilibrium(): rho, ux, uy need %d values to match f (9, %d)" % (n, n))
+    if n == 0:
+        return None
+    d_rho, d_ux, d_uy, d_f = (_device(a) for a in args)
+    kernels.equilibrium(d_rho.reshape(-1), d_ux.reshape(-1), d_uy.reshape(-1), out=d_f)
+    if isinstance(f, np.ndarray):
+        _staging.write_back(f, d_f)
+    return None
+
+
+def collide(f_ikl, omega):
+    """BGK collide in place on ``f_ikl`` (9, N) with relaxation ``omega``."""
+    _check_array(f_ikl, "collide", 2)
+    try:
+        omega = float(omega)
+    except (TypeError, ValueError):
+        raise _incompatible("collide") from None
+    if f_ikl.shape[1] == 0:
+        return None
+    d_f = _device(f_ikl)
+    kernels.collide_(d_f, omega)
+    if isinstance(f_ikl, np.ndarray):
+        _staging.write_back(f_ikl, d_f)
+    return None

@@ -1,0 +1,41 @@
+"""Host <-> device staging for the drop-in PyLB functions.
+
+numpy arguments are copied to the GPU, processed there and written back into
+the caller's buffer, which keeps the reference's in-place semantics.  torch
+tensors already on a ROCm device are used as they are.
+"""
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+
+_TORCH_OF = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32}
+
+
+def is_device_tensor(a):
+    return isinstance(a, torch.Tensor) and a.device.type == "cuda"
+
+
+def device():
+    nat.require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_device(a):
+    """A device tensor holding the values of numpy array ``a`` (any dtype)."""
+    a = np.ascontiguousarray(a)
+    if a.dtype in _TORCH_OF:
+        host = torch.from_numpy(a)
+    else:
+        # byte-preserving view for dtypes torch does not know (streaming only)
+        host = torch.from_numpy(a.view({1: np.uint8, 2: np.int16, 4: np.int32, 8: np.int64}[a.itemsize]))
+    return host.to(device())
+
+
+def write_back(dst, t):
+    """Copy device tensor ``t`` into numpy array ``dst`` in place."""
+    host = t.cpu().numpy()
+    if host.dtype != dst.dtype:
+        host = host.view(dst.dtype)
+    dst[...] = host.reshape(dst.shape)

@@ -1,0 +1,104 @@
+"""Loader for ``liblbm_d2q9.so``, the C ABI declared in ``include/lbm_d2q9.h``.
+
+There is no fallback: if the library is missing or no GPU is visible, the
+calls raise.  ``torch`` is imported before the library is opened so that the
+library binds to the HIP runtime torch has already loaded (both are the
+``libamdhip64.so.7`` soname); device pointers and streams are then shared.
+"""
+
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm_d2q9.so")
+
+LBM_OK = 0
+_ERRORS = {
+    -1: (ValueError, "a required pointer is NULL"),
+    -2: (ValueError, "size out of range"),
+    -3: (ValueError, "argument out of range"),
+    -4: (ValueError, "input and output buffers overlap"),
+    -5: (ValueError, "workspace too small"),
+}
+
+BC_PERIODIC = 0
+BC_CAVITY = 1
+FMT_F32, FMT_BF16, FMT_F16 = 0, 1, 2
+
+_vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
+_f64, _f32 = ctypes.c_double, ctypes.c_float
+
+# name -> (restype, argtypes); keep in step with include/lbm_d2q9.h
+SIGNATURES = {
+    "lbm_version": (ctypes.c_char_p, []),
+    "lbm_step_variant_count": (_i32, []),
+    "lbm_equilibrium_f64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "lbm_equilibrium_f32": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "lbm_collide_f64": (_i32, [_vp, _i64, _f64, _vp, _vp, _vp, _vp]),
+    "lbm_collide_f32": (_i32, [_vp, _i64, _f32, _vp, _vp, _vp, _vp]),
+    "lbm_moments_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "lbm_moments_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "lbm_stream_periodic": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp]),
+    "lbm_stream_bounce_back_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp]),
+    "lbm_stream_bounce_back_f32": (_i32, [_vp, _vp, _i64, _i64, _f32, _vp]),
+    "lbm_step_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _i32, _f64, _vp, _vp, _vp, _i32, _vp]),
+    "lbm_step_f32": (_i32, [_vp, _vp, _i64, _i64, _f32, _i32, _f32, _vp, _vp, _vp, _i32, _vp]),
+    "lbm_reduce_workspace_bytes": (_i64, []),
+    "lbm_sum_f64": (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    "lbm_sum_f32": (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    "lbm_velocity_residual_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Open the library (once) and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            "liblbm_d2q9.so is not built; run `python -m latticeboltzmann_amd.build` "
+            "(the HIP kernels are the only implementation)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, what):
+    """Raise for a non-zero status returned by a C-ABI call."""
+    if status == LBM_OK:
+        return
+    if status in _ERRORS:
+        exc, msg = _ERRORS[status]
+        raise exc("%s: %s" % (what, msg))
+    raise RuntimeError("%s: HIP error %d" % (what, status))
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("latticeboltzmann_amd needs a ROCm GPU; none is visible")
+
+
+def stream_ptr(stream=None):
+    """hipStream_t of ``stream`` (default: torch's current stream) as an int."""
+    s = torch.cuda.current_stream() if stream is None else stream
+    return s.cuda_stream
+
+
+def ptr(t):
+    """Device address of a tensor (or None for None)."""
+    return None if t is None else t.data_ptr()
+
+
+def call(name, *args):
+    """Call a C-ABI entry point and raise on a non-zero status."""
+    status = getattr(load(), name)(*args)
+    check(status, name)

@@ -1,0 +1,74 @@
+"""Build recipe for ``liblbm_d2q9.so`` (hand-written HIP kernels + C ABI).
+
+Each ``csrc/*.hip`` translation unit is compiled for gfx950 with its own
+flags and the objects are linked into one shared library next to this file.
+
+* ``d2q9_kernels.hip`` (the bit-exact FP64/FP32 path) is compiled with
+  ``-ffp-contract=off``: numpy never fuses a multiply into an add, so neither
+  may the kernels that must reproduce it.
+* every unit is compiled with the ``packed-fp32-ops`` target feature switched
+  off, so the compiler never emits packed FP32 arithmetic.
+
+Usage: ``python -m latticeboltzmann_amd.build`` (or ``build()`` from Python).
+"""
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB_NAME = "liblbm_d2q9.so"
+LIB_PATH = os.path.join(HERE, LIB_NAME)
+BUILD_DIR = os.path.join(HERE, "_build")
+ARCH = "gfx950"
+
+COMMON = [
+    "--offload-arch=" + ARCH, "-O3", "-fPIC", "-std=c++17",
+    "-Wall", "-Wno-unused-function",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+# per-unit flags
+UNITS = {
+    "d2q9_kernels.hip": ["-ffp-contract=off"],
+    "d2q9_lowprec.hip": ["-ffp-contract=fast"],
+    "d2q9_batch.hip": ["-ffp-contract=off"],
+    "surrogate.hip": ["-ffp-contract=fast"],
+}
+
+
+def hipcc():
+    exe = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(exe):
+        raise RuntimeError("hipcc not found: the HIP toolchain is required to build " + LIB_NAME)
+    return exe
+
+
+def build(verbose=False):
+    """Compile every HIP unit for gfx950 and link ``liblbm_d2q9.so``."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    cc = hipcc()
+    objs = []
+    for unit, flags in UNITS.items():
+        src = os.path.join(CSRC, unit)
+        if not os.path.exists(src):
+            continue
+        obj = os.path.join(BUILD_DIR, unit.replace(".hip", ".o"))
+        cmd = [cc, *COMMON, *flags, "-c", src, "-o", obj]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB_PATH + ".tmp"
+    cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB_PATH)

@@ -1,0 +1,544 @@
+// D2Q9 BGK kernels for gfx950 in FP64 and FP32: equilibrium, collide,
+// periodic streaming, streaming with cavity walls, the fused pull
+// stream-collide step, and deterministic FP64 reductions.
+//
+// Built with -ffp-contract=off: every product is rounded before it is added,
+// as in numpy, so the FP64 results match the reference's numpy opt1 path bit
+// for bit (cavity_opt1.py:100-245).
+//
+// Layout: f[i][k][l] (channel, x, y) with y contiguous.  One thread owns
+// V nodes of one x-row, V*kBlock consecutive y values per block, so every
+// load and store instruction of a wave touches 64 consecutive elements of one
+// plane.  The pull of channel i reads plane i at (k - cx_i, l - cy_i): whole
+// rows k-1, k, k+1 shifted by at most one element, so the reads stay
+// coalesced and every population is read exactly once per step.
+
+#include "d2q9_node.hpp"
+#include "lbm_internal.hpp"
+
+#pragma clang fp contract(off)
+
+namespace lbm {
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld(const T *p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st(T *p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+// Row base pointers of the nine pulls for row k.
+template <typename T>
+struct PullRows {
+    const T *p[9];
+    __device__ __forceinline__ PullRows(const T *src, int64_t plane, int k, int nx, int ny) {
+        const int km = k == 0 ? nx - 1 : k - 1;
+        const int kp = k == nx - 1 ? 0 : k + 1;
+        const int64_t rk = int64_t(k) * ny, rm = int64_t(km) * ny, rp = int64_t(kp) * ny;
+        p[0] = src + rk;
+        p[1] = src + plane + rm;          // E  pulls from (k-1, l)
+        p[2] = src + 2 * plane + rk;      // N  pulls from (k,   l-1)
+        p[3] = src + 3 * plane + rp;      // W  pulls from (k+1, l)
+        p[4] = src + 4 * plane + rk;      // S  pulls from (k,   l+1)
+        p[5] = src + 5 * plane + rm;      // NE pulls from (k-1, l-1)
+        p[6] = src + 6 * plane + rp;      // NW pulls from (k+1, l-1)
+        p[7] = src + 7 * plane + rp;      // SW pulls from (k+1, l+1)
+        p[8] = src + 8 * plane + rm;      // SE pulls from (k-1, l+1)
+    }
+};
+
+template <bool NT, typename T>
+__device__ __forceinline__ void pull9(const PullRows<T> &r, int l, int ny, T g[9]) {
+    const int lm = l == 0 ? ny - 1 : l - 1;
+    const int lp = l == ny - 1 ? 0 : l + 1;
+    g[0] = ld<NT>(r.p[0] + l);
+    g[1] = ld<NT>(r.p[1] + l);
+    g[2] = ld<NT>(r.p[2] + lm);
+    g[3] = ld<NT>(r.p[3] + l);
+    g[4] = ld<NT>(r.p[4] + lp);
+    g[5] = ld<NT>(r.p[5] + lm);
+    g[6] = ld<NT>(r.p[6] + lm);
+    g[7] = ld<NT>(r.p[7] + lp);
+    g[8] = ld<NT>(r.p[8] + lp);
+}
+
+// Cavity walls on the pulled values g of node (k, l), applied in the order of
+// cavity_opt1.py:201-225: bottom, lid (density from the post-bottom values),
+// left, right.  c points at the node's own pre-stream value of channel 0.
+template <typename T>
+__device__ __forceinline__ void cavity_walls(T g[9], const T *c, int64_t plane, int k, int l,
+                                             int nx, int ny, T u0) {
+    if (l == 0) {  // bottom: N <- S, NE <- SW, NW <- SE
+        g[2] = c[4 * plane];
+        g[5] = c[7 * plane];
+        g[6] = c[8 * plane];
+    }
+    if (l == ny - 1) {  // moving lid
+        const T c2 = c[2 * plane], c5 = c[5 * plane], c6 = c[6 * plane];
+        T rl = c6 + c2;
+        rl = rl + c5;
+        rl = rl + g[6];
+        rl = rl + g[2];
+        rl = rl + g[5];
+        rl = rl + g[3];
+        rl = rl + g[0];
+        rl = rl + g[1];
+        const T t = (Weights<T>::lid * rl) * u0;
+        g[4] = c2;
+        g[8] = c6 + t;
+        g[7] = c5 - t;
+    }
+    if (k == 0) {  // left: E <- W, NE <- SW, SE <- NW
+        g[1] = c[3 * plane];
+        g[5] = c[7 * plane];
+        g[8] = c[6 * plane];
+    }
+    if (k == nx - 1) {  // right: W <- E, NW <- SE, SW <- NE
+        g[3] = c[1 * plane];
+        g[6] = c[8 * plane];
+        g[7] = c[5 * plane];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused step: pull (+ walls) + collide, src -> dst.
+// grid = nx * nseg blocks, nseg = ceil(ny / (V * kBlock)).
+template <typename T, int BC, bool MOM, int V, bool NT>
+__global__ __launch_bounds__(kBlock) void k_step(const T *__restrict__ src, T *__restrict__ dst,
+                                                 int nx, int ny, int64_t plane, int nseg, T omega,
+                                                 T u0, T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                                 T *__restrict__ uy_o) {
+    const int b = blockIdx.x;
+    const int k = b / nseg;
+    const int seg = b - k * nseg;
+    const PullRows<T> rows(src, plane, k, nx, ny);
+    const int64_t rk = int64_t(k) * ny;
+    const int l0 = seg * (V * kBlock) + threadIdx.x;
+
+    T g[V][9];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int l = l0 + v * kBlock;
+        if (l < ny) pull9<NT>(rows, l, ny, g[v]);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int l = l0 + v * kBlock;
+        if (l >= ny) continue;
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g[v], src + rk + l, plane, k, l, nx, ny, u0);
+        T rho, ux, uy;
+        collide_opt1(g[v], omega, rho, ux, uy);
+        T *d = dst + rk + l;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) st<NT>(d + i * plane, g[v][i]);
+        if constexpr (MOM) {
+            rho_o[rk + l] = rho;
+            ux_o[rk + l] = ux;
+            uy_o[rk + l] = uy;
+        }
+    }
+}
+
+// Streaming with cavity walls, no collide (stream_and_bounce_back).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_stream_bounce(const T *__restrict__ src,
+                                                          T *__restrict__ dst, int nx, int ny,
+                                                          int64_t plane, int nseg, T u0) {
+    const int b = blockIdx.x;
+    const int k = b / nseg;
+    const int l = (b - k * nseg) * kBlock + threadIdx.x;
+    if (l >= ny) return;
+    const PullRows<T> rows(src, plane, k, nx, ny);
+    const int64_t rk = int64_t(k) * ny;
+    T g[9];
+    pull9<false>(rows, l, ny, g);
+    cavity_walls(g, src + rk + l, plane, k, l, nx, ny, u0);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dst[i * plane + rk + l] = g[i];
+}
+
+// Periodic streaming of U-sized elements (a pure permutation, PyLB.stream).
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_stream_periodic(const U *__restrict__ src,
+                                                            U *__restrict__ dst, int nx, int ny,
+                                                            int64_t plane, int nseg) {
+    const int b = blockIdx.x;
+    const int k = b / nseg;
+    const int l = (b - k * nseg) * kBlock + threadIdx.x;
+    if (l >= ny) return;
+    const PullRows<U> rows(src, plane, k, nx, ny);
+    U g[9];
+    pull9<false>(rows, l, ny, g);
+    const int64_t rk = int64_t(k) * ny;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dst[i * plane + rk + l] = g[i];
+}
+
+// Collide in place on f[9][n] (PyLB.collide).
+template <typename T, bool MOM>
+__global__ __launch_bounds__(kBlock) void k_collide(T *__restrict__ f, int64_t n, T omega,
+                                                    T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                                    T *__restrict__ uy_o) {
+    const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    T g[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = f[i * n + j];
+    T rho, ux, uy;
+    collide_opt1(g, omega, rho, ux, uy);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i * n + j] = g[i];
+    if constexpr (MOM) {
+        rho_o[j] = rho;
+        ux_o[j] = ux;
+        uy_o[j] = uy;
+    }
+}
+
+// Moments of f[9][n] in the opt1 order (cavity_opt1.py:156-158).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_moments(const T *__restrict__ f, int64_t n,
+                                                    T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                                    T *__restrict__ uy_o) {
+    const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    T g[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = f[i * n + j];
+    T rho, ux, uy;
+    moments_opt1(g, rho, ux, uy);
+    rho_o[j] = rho;
+    ux_o[j] = ux;
+    uy_o[j] = uy;
+}
+
+// f[9][n] = f_eq(rho, ux, uy) (PyLB.equilibrium).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_equilibrium(const T *__restrict__ rho,
+                                                        const T *__restrict__ ux,
+                                                        const T *__restrict__ uy,
+                                                        T *__restrict__ f, int64_t n) {
+    const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (j >= n) return;
+    T feq[9];
+    equilibrium_opt1(rho[j], ux[j], uy[j], feq);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i * n + j] = feq[i];
+}
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: pass 1 with a fixed grid of kReduceBlocks blocks,
+// each thread accumulating a grid-strided sequence with Neumaier compensation,
+// then a fixed-shape tree in LDS; pass 2 folds the partials in one block.
+
+__device__ __forceinline__ void neumaier_add(double &s, double &c, double x) {
+    const double t = s + x;
+    if (fabs(s) >= fabs(x)) c += (s - t) + x;
+    else c += (x - t) + s;
+    s = t;
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
+        __syncthreads();
+    }
+    return red[0];
+}
+
+__device__ __forceinline__ double block_max(double v) {
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = v;
+    __syncthreads();
+#pragma unroll
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    return red[0];
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_sum_pass1(const T *__restrict__ x, int64_t n,
+                                                      double *__restrict__ partial) {
+    double s = 0.0, c = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride)
+        neumaier_add(s, c, double(x[j]));
+    const double tot = block_sum(s + c);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_pass2(const double *__restrict__ partial, int np,
+                                                      double *__restrict__ out) {
+    double s = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < np; j += kBlock) neumaier_add(s, c, partial[j]);
+    const double tot = block_sum(s + c);
+    if (threadIdx.x == 0) out[0] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_pass2(const double *__restrict__ partial, int np,
+                                                      double *__restrict__ out) {
+    double m = 0.0;
+    for (int j = threadIdx.x; j < np; j += kBlock) m = fmax(m, partial[j]);
+    const double tot = block_max(m);
+    if (threadIdx.x == 0) out[0] = tot;
+}
+
+__global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restrict__ f, int64_t n,
+                                                           double *__restrict__ uxp,
+                                                           double *__restrict__ uyp,
+                                                           double *__restrict__ partial) {
+    double m = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride) {
+        double g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = f[i * n + j];
+        double rho, ux, uy;
+        moments_opt1(g, rho, ux, uy);
+        const double dx = ux - uxp[j], dy = uy - uyp[j];
+        m = fmax(m, sqrt(dx * dx + dy * dy));
+        uxp[j] = ux;
+        uyp[j] = uy;
+    }
+    const double tot = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+
+struct StepShape {
+    int v;
+    bool nt;
+};
+// Launch shapes of the fused step, indexed by `variant`.
+constexpr StepShape kStepShapes[] = {{1, false}, {2, false}, {4, false},
+                                     {1, true},  {2, true},  {4, true}};
+constexpr int kNumStepShapes = int(sizeof(kStepShapes) / sizeof(kStepShapes[0]));
+constexpr int kDefaultStepShape = 1;
+
+template <typename T, int BC, bool MOM, int V, bool NT>
+void launch_step_v(const T *src, T *dst, int nx, int ny, T omega, T u0, T *rho, T *ux, T *uy,
+                   hipStream_t s) {
+    const int nseg = (ny + V * kBlock - 1) / (V * kBlock);
+    const int64_t plane = int64_t(nx) * ny;
+    hipLaunchKernelGGL((k_step<T, BC, MOM, V, NT>), dim3(unsigned(int64_t(nx) * nseg)),
+                       dim3(kBlock), 0, s, src, dst, nx, ny, plane, nseg, omega, u0, rho, ux, uy);
+}
+
+template <typename T, int BC, bool MOM>
+void launch_step_bc(const T *src, T *dst, int nx, int ny, T omega, T u0, T *rho, T *ux, T *uy,
+                    int shape, hipStream_t s) {
+    switch (shape) {
+        case 0: launch_step_v<T, BC, MOM, 1, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        case 1: launch_step_v<T, BC, MOM, 2, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        case 2: launch_step_v<T, BC, MOM, 4, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        case 3: launch_step_v<T, BC, MOM, 1, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        case 4: launch_step_v<T, BC, MOM, 2, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        default: launch_step_v<T, BC, MOM, 4, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+    }
+}
+
+template <typename T>
+int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, T omega, int bc, T u0, T *rho, T *ux,
+              T *uy, int variant, void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
+    if (variant < -1 || variant >= kNumStepShapes) return LBM_E_ARG;
+    const bool mom = rho != nullptr;
+    if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
+    const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
+    if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
+    const int64_t nseg_max = (ny + kBlock - 1) / kBlock;
+    if (nx * nseg_max > 0x7fffffffLL) return LBM_E_SIZE;
+    const int shape = variant < 0 ? kDefaultStepShape : variant;
+    hipStream_t s = as_stream(stream);
+    const int inx = int(nx), iny = int(ny);
+    if (bc == LBM_BC_CAVITY) {
+        if (mom) launch_step_bc<T, LBM_BC_CAVITY, true>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+        else launch_step_bc<T, LBM_BC_CAVITY, false>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+    } else {
+        if (mom) launch_step_bc<T, LBM_BC_PERIODIC, true>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+        else launch_step_bc<T, LBM_BC_PERIODIC, false>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+    }
+    return launch_status();
+}
+
+template <typename T>
+int stream_bounce_impl(const T *src, T *dst, int64_t nx, int64_t ny, T u0, void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
+    if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
+    const int64_t nseg = (ny + kBlock - 1) / kBlock;
+    if (nx * nseg > 0x7fffffffLL) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_stream_bounce<T>, dim3(unsigned(nx * nseg)), dim3(kBlock), 0,
+                       as_stream(stream), src, dst, int(nx), int(ny), nx * ny, int(nseg), u0);
+    return launch_status();
+}
+
+template <typename U>
+void launch_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, int64_t nseg,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_stream_periodic<U>, dim3(unsigned(nx * nseg)), dim3(kBlock), 0, s,
+                       static_cast<const U *>(src), static_cast<U *>(dst), int(nx), int(ny),
+                       nx * ny, int(nseg));
+}
+
+template <typename T>
+int collide_impl(T *f, int64_t n, T omega, T *rho, T *ux, T *uy, void *stream) {
+    if (!f) return LBM_E_NULL;
+    if (n <= 0 || n > (int64_t(1) << 40)) return LBM_E_SIZE;
+    const bool mom = rho != nullptr;
+    if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 0x7fffffffLL) return LBM_E_SIZE;
+    if (mom)
+        hipLaunchKernelGGL((k_collide<T, true>), dim3(unsigned(nb)), dim3(kBlock), 0,
+                           as_stream(stream), f, n, omega, rho, ux, uy);
+    else
+        hipLaunchKernelGGL((k_collide<T, false>), dim3(unsigned(nb)), dim3(kBlock), 0,
+                           as_stream(stream), f, n, omega, rho, ux, uy);
+    return launch_status();
+}
+
+template <typename T>
+int moments_impl(const T *f, int64_t n, T *rho, T *ux, T *uy, void *stream) {
+    if (!f || !rho || !ux || !uy) return LBM_E_NULL;
+    if (n <= 0 || n > (int64_t(1) << 40)) return LBM_E_SIZE;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 0x7fffffffLL) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_moments<T>, dim3(unsigned(nb)), dim3(kBlock), 0, as_stream(stream), f, n,
+                       rho, ux, uy);
+    return launch_status();
+}
+
+template <typename T>
+int equilibrium_impl(const T *rho, const T *ux, const T *uy, T *f, int64_t n, void *stream) {
+    if (!rho || !ux || !uy || !f) return LBM_E_NULL;
+    if (n <= 0 || n > (int64_t(1) << 40)) return LBM_E_SIZE;
+    const int64_t nb = (n + kBlock - 1) / kBlock;
+    if (nb > 0x7fffffffLL) return LBM_E_SIZE;
+    const int64_t fb = 9 * n * int64_t(sizeof(T)), sb = n * int64_t(sizeof(T));
+    if (overlaps(f, fb, rho, sb) || overlaps(f, fb, ux, sb) || overlaps(f, fb, uy, sb))
+        return LBM_E_ALIAS;
+    hipLaunchKernelGGL(k_equilibrium<T>, dim3(unsigned(nb)), dim3(kBlock), 0, as_stream(stream),
+                       rho, ux, uy, f, n);
+    return launch_status();
+}
+
+template <typename T>
+int sum_impl(const T *x, int64_t n, double *work, double *out, void *stream) {
+    if (!x || !work || !out) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_sum_pass1<T>, dim3(kReduceBlocks), dim3(kBlock), 0, s, x, n, work);
+    hipLaunchKernelGGL(k_sum_pass2, dim3(1), dim3(kBlock), 0, s, work, kReduceBlocks, out);
+    return launch_status();
+}
+
+}  // namespace lbm
+
+using namespace lbm;
+
+extern "C" {
+
+const char *lbm_version(void) { return "latticeboltzmann_amd 0.1.0 gfx950"; }
+int lbm_step_variant_count(void) { return kNumStepShapes; }
+
+int lbm_equilibrium_f64(const double *rho, const double *ux, const double *uy, double *f,
+                        int64_t n, void *stream) {
+    return equilibrium_impl(rho, ux, uy, f, n, stream);
+}
+int lbm_equilibrium_f32(const float *rho, const float *ux, const float *uy, float *f, int64_t n,
+                        void *stream) {
+    return equilibrium_impl(rho, ux, uy, f, n, stream);
+}
+
+int lbm_collide_f64(double *f, int64_t n, double omega, double *rho, double *ux, double *uy,
+                    void *stream) {
+    return collide_impl(f, n, omega, rho, ux, uy, stream);
+}
+int lbm_collide_f32(float *f, int64_t n, float omega, float *rho, float *ux, float *uy,
+                    void *stream) {
+    return collide_impl(f, n, omega, rho, ux, uy, stream);
+}
+
+int lbm_moments_f64(const double *f, int64_t n, double *rho, double *ux, double *uy,
+                    void *stream) {
+    return moments_impl(f, n, rho, ux, uy, stream);
+}
+int lbm_moments_f32(const float *f, int64_t n, float *rho, float *ux, float *uy, void *stream) {
+    return moments_impl(f, n, rho, ux, uy, stream);
+}
+
+int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,
+                        void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+        return LBM_E_ARG;
+    const int64_t bytes = 9 * nx * ny * elem_bytes;
+    if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
+    const int64_t nseg = (ny + kBlock - 1) / kBlock;
+    if (nx * nseg > 0x7fffffffLL) return LBM_E_SIZE;
+    hipStream_t s = as_stream(stream);
+    switch (elem_bytes) {
+        case 1: launch_stream_periodic<uint8_t>(src, dst, nx, ny, nseg, s); break;
+        case 2: launch_stream_periodic<uint16_t>(src, dst, nx, ny, nseg, s); break;
+        case 4: launch_stream_periodic<uint32_t>(src, dst, nx, ny, nseg, s); break;
+        default: launch_stream_periodic<uint64_t>(src, dst, nx, ny, nseg, s); break;
+    }
+    return launch_status();
+}
+
+int lbm_stream_bounce_back_f64(const double *src, double *dst, int64_t nx, int64_t ny, double u0,
+                               void *stream) {
+    return stream_bounce_impl(src, dst, nx, ny, u0, stream);
+}
+int lbm_stream_bounce_back_f32(const float *src, float *dst, int64_t nx, int64_t ny, float u0,
+                               void *stream) {
+    return stream_bounce_impl(src, dst, nx, ny, u0, stream);
+}
+
+int lbm_step_f64(const double *src, double *dst, int64_t nx, int64_t ny, double omega, int bc,
+                 double u0, double *rho, double *ux, double *uy, int variant, void *stream) {
+    return step_impl(src, dst, nx, ny, omega, bc, u0, rho, ux, uy, variant, stream);
+}
+int lbm_step_f32(const float *src, float *dst, int64_t nx, int64_t ny, float omega, int bc,
+                 float u0, float *rho, float *ux, float *uy, int variant, void *stream) {
+    return step_impl(src, dst, nx, ny, omega, bc, u0, rho, ux, uy, variant, stream);
+}
+
+int64_t lbm_reduce_workspace_bytes(void) { return int64_t(kReduceBlocks) * int64_t(sizeof(double)); }
+
+int lbm_sum_f64(const double *x, int64_t n, double *work, double *out, void *stream) {
+    return sum_impl(x, n, work, out, stream);
+}
+int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stream) {
+    return sum_impl(x, n, work, out, stream);
+}
+
+int lbm_velocity_residual_f64(const double *f, int64_t n, double *ux_prev, double *uy_prev,
+                              double *work, double *out, void *stream) {
+    if (!f || !ux_prev || !uy_prev || !work || !out) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_residual_pass1, dim3(kReduceBlocks), dim3(kBlock), 0, s, f, n, ux_prev,
+                       uy_prev, work);
+    hipLaunchKernelGGL(k_max_pass2, dim3(1), dim3(kBlock), 0, s, work, kReduceBlocks, out);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+"""Times the CPU oracle on one core: the CPU baseline that bench.py reports.
+
+TEST / MEASUREMENT INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Runs in its own process pinned to one core, with BLAS/OpenMP threads set to
+one, so the figure is a single-core rate.  Two legs:
+
+* ``numpy``: the numpy opt1 restatement (cavity_opt1.py:100-245), the
+  reference's optimised pure-numpy flavour;
+* ``c``: the single-thread C restatement of the same step (-O2).
+
+Prints one JSON object: MLUPS per leg, steps timed, grid, core count used.
+"""
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+for var in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+    os.environ[var] = "1"
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from oracle import ctypes_ref, d2q9_numpy  # noqa: E402
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def time_leg(fn, f, budget_s, max_steps):
+    """Run single steps until ``budget_s`` seconds or ``max_steps`` steps."""
+    steps, t0 = 0, time.perf_counter()
+    while steps < max_steps:
+        fn(f)
+        steps += 1
+        if time.perf_counter() - t0 >= budget_s:
+            break
+    return steps, time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--omega", type=float, default=0.897343)
+    ap.add_argument("--budget", type=float, default=8.0, help="seconds per leg")
+    ap.add_argument("--max-steps", type=int, default=50)
+    args = ap.parse_args()
+
+    cores = 1
+    try:
+        os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        cores = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        pass
+
+    n = args.n
+    f = d2q9_numpy.cavity_init(n, n)
+    # warm the page cache / allocator with one step of each leg
+    d2q9_numpy.cavity_step(f, args.omega, 0.1)
+    steps_np, t_np = time_leg(lambda g: d2q9_numpy.cavity_step(g, args.omega, 0.1), f,
+                              args.budget, args.max_steps)
+
+    ctypes_ref.lib()
+    g = np.ascontiguousarray(f)
+    h = np.empty_like(g)
+
+    def c_step(state):
+        a, b = state
+        ctypes_ref.lib().oracle_step_f64(ctypes_ref._ptr(a), ctypes_ref._ptr(b), n, n,
+                                         args.omega, 1, 0.1, None, None, None)
+        state.reverse()
+
+    state = [g, h]
+    c_step(state)
+    steps_c, t_c = time_leg(c_step, state, args.budget, args.max_steps)
+
+    print(json.dumps({
+        "grid": [n, n],
+        "numpy_opt1": {"steps": steps_np, "seconds": t_np,
+                       "mlups": n * n * steps_np / t_np / 1e6},
+        "c_port": {"steps": steps_c, "seconds": t_c, "mlups": n * n * steps_c / t_c / 1e6},
+        "cores": cores,
+        "cpu_count": os.cpu_count(),
+        "cpu_model": cpu_model(),
+    }))
+
+
+if __name__ == "__main__":
+    main()

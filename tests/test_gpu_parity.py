@@ -1,0 +1,278 @@
+"""GPU parity of the HIP kernels against the CPU oracle (bit-exact in FP64/FP32).
+
+Every call goes through the C ABI (latticeboltzmann_amd.kernels / PyLB).
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ctypes_ref as cref
+from oracle import d2q9_numpy as o
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(12345)
+
+SHAPES = [(1, 1), (1, 7), (7, 1), (2, 2), (5, 7), (33, 300), (64, 520), (17, 1030)]
+
+
+def populations(nx, ny, dtype=np.float64, spread=0.2):
+    f = o.weights(np.float64)[:, None, None] * (1 + spread * RNG.standard_normal((9, nx, ny)))
+    return f.astype(dtype)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def host(t):
+    return t.cpu().numpy()
+
+
+# ---------------------------------------------------------------------------
+# streaming (a permutation: bit-exact for every element size)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape", [(1, 1), (1, 9), (7, 5), (300, 257), (64, 512)])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.float16, np.uint8, np.int16])
+def test_stream_bitexact(gpu, shape, dtype):
+    from latticeboltzmann_amd import PyLB
+    a = (RNG.random((9,) + shape) * 200).astype(dtype)
+    expect = a.copy()
+    o.stream(expect)
+    got = a.copy()
+    PyLB.stream(got)
+    assert np.array_equal(got.view(np.uint8), expect.view(np.uint8))
+
+
+def test_stream_device_tensor_in_place(gpu):
+    from latticeboltzmann_amd import PyLB
+    a = RNG.random((9, 40, 70))
+    t = dev(a)
+    PyLB.stream(t)
+    o.stream(a)
+    assert np.array_equal(host(t), a)
+
+
+def test_stream_periodic_return_kat_on_gpu(gpu):
+    # tests/simpleFlowsTest.py:176-196 on the device
+    from latticeboltzmann_amd import kernels
+    g = torch.zeros((9, 9, 9), dtype=torch.float64, device=gpu)
+    g[:, 4, 4] = 1
+    for _ in range(9):
+        g = kernels.stream(g)
+    assert (g[:, 4, 4] == 1).all() and float(g.sum()) == 9
+
+
+# ---------------------------------------------------------------------------
+# walls
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("u0", [0.0, 0.1])
+def test_stream_and_bounce_back_bitexact(gpu, shape, dtype, u0):
+    from latticeboltzmann_amd import PyLB
+    f = populations(*shape, dtype)
+    expect = f.copy()
+    o.stream_and_bounce_back(expect, u0)
+    got = f.copy()
+    PyLB.stream_and_bounce_back(got, u0)
+    assert np.array_equal(got, expect)
+
+
+# ---------------------------------------------------------------------------
+# equilibrium / collide (the reference's 02-CollideTest, restated, plus bitwise)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_equilibrium_array_overload(gpu, dtype):
+    from latticeboltzmann_amd import PyLB
+    rho = np.abs(RNG.random((2, 2))).astype(dtype)
+    ux = RNG.random((2, 2)).astype(dtype)
+    uy = RNG.random((2, 2)).astype(dtype)
+    e1 = np.zeros((9, 2, 2), dtype=dtype)
+    assert PyLB.equilibrium(rho.reshape(-1), ux.reshape(-1), uy.reshape(-1), e1.reshape(9, -1)) is None
+    assert np.array_equal(e1, o.equilibrium(rho, ux, uy))
+    if dtype == np.float64:  # tests/02-CollideTest.py:94-102
+        e2 = o.equilibrium_opt0(rho, np.array([ux, uy]))
+        assert np.abs(e1 - e2).max() < 1e-7
+
+
+def test_equilibrium_scalar_overload_is_float32(gpu):
+    from latticeboltzmann_amd import PyLB
+    e = PyLB.equilibrium(1.05, 0.03, -0.02)
+    assert e.shape == (9,) and e.dtype == np.float32
+    one = np.ones(1, np.float32)
+    expect = o.equilibrium(one * np.float32(1.05), one * np.float32(0.03), one * np.float32(-0.02))
+    assert np.array_equal(e, expect[:, 0])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("omega", [0.5, 1.0, 1.7])
+def test_collide_bitexact(gpu, dtype, omega):
+    from latticeboltzmann_amd import PyLB
+    f = np.abs(RNG.random((9, 4, 4))).astype(dtype)
+    c1, c2 = f.copy(), f.copy()
+    assert PyLB.collide(c1.reshape(9, -1), omega) is None
+    o.collide(c2, dtype(omega))
+    assert np.array_equal(c1, c2)
+    if dtype == np.float64:  # tests/02-CollideTest.py:104-111
+        c3 = f.copy()
+        o.collide_opt0(c3, omega)
+        assert np.abs(c1 - c3).max() < 1e-7
+
+
+def test_collide_moments_and_large_n(gpu):
+    from latticeboltzmann_amd import kernels
+    f = populations(300, 301)
+    t = dev(f)
+    rho, ux, uy = kernels.collide_(t.view(9, -1), 1.3, moments=True)
+    g = f.copy()
+    mom = o.collide(g, 1.3)
+    assert np.array_equal(host(t), g)
+    for a, b in zip((rho, ux, uy), mom):
+        assert np.array_equal(host(a).reshape(b.shape), b)
+
+
+def test_moments_bitexact(gpu):
+    from latticeboltzmann_amd import kernels
+    f = populations(50, 70)
+    rho, ux, uy = kernels.moments(dev(f))
+    for a, b in zip((rho, ux, uy), o.moments(f)):
+        assert np.array_equal(host(a), b)
+
+
+# ---------------------------------------------------------------------------
+# fused step: every launch variant, both walls kinds, both dtypes
+# ---------------------------------------------------------------------------
+
+def variants():
+    from latticeboltzmann_amd import _native
+    return range(_native.load().lbm_step_variant_count())
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("bc", ["cavity", "periodic"])
+def test_fused_step_bitexact_all_variants(gpu, shape, dtype, bc):
+    from latticeboltzmann_amd import _native, kernels
+    f = populations(*shape, dtype)
+    expect, rho_e, ux_e, uy_e = cref.step(f, 1.7, bc, 0.1)
+    src = dev(f)
+    kind = _native.BC_CAVITY if bc == "cavity" else _native.BC_PERIODIC
+    for v in variants():
+        dst = torch.empty_like(src)
+        mom = tuple(torch.empty(shape, dtype=src.dtype, device=gpu) for _ in range(3))
+        kernels.step(src, dst, 1.7, kind, 0.1, moments=mom, variant=v)
+        assert np.array_equal(host(dst), expect), "variant %d" % v
+        for a, b in zip(mom, (rho_e, ux_e, uy_e)):
+            assert np.array_equal(host(a), b), "variant %d moments" % v
+        dst2 = torch.empty_like(src)
+        kernels.step(src, dst2, 1.7, kind, 0.1, variant=v)
+        assert torch.equal(dst, dst2)
+
+
+def test_dropin_cavity_step_sequence_bitexact(gpu):
+    # the opt2 driver step (cavity_opt2.py:276-277) through the drop-in API
+    from latticeboltzmann_amd import PyLB
+    f = o.cavity_init(24, 20)
+    g = f.copy()
+    for _ in range(5):
+        PyLB.stream_and_bounce_back(f, 0.1)
+        PyLB.collide(f.reshape(9, -1), 1.529988)
+        o.cavity_step(g, 1.529988, 0.1)
+    assert np.array_equal(f, g)
+
+
+def test_cavity_crop_512_bitexact_200_steps(gpu):
+    # the full-step comparison of the 2048^2 cavity workload on its 512^2 crop:
+    # Re = 1000, U = 0.1 -> omega = 1/(3 * 0.1 * 512/1000 + 1/2)
+    from latticeboltzmann_amd.lattice import Lattice, omega_for_cavity
+    n, steps = 512, 200
+    omega = omega_for_cavity(n, 1000.0, 0.1)
+    assert abs(omega - 1.529988) < 1e-6
+    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64)
+    f0 = lat.populations()
+    assert np.array_equal(f0, o.cavity_init(n, n))
+    lat.step(steps)
+    expect = cref.run(f0, omega, steps, "cavity", 0.1)
+    got = lat.populations()
+    assert np.array_equal(got, expect)
+    # the contract's tolerance, stated: full steps within 1e-12 relative
+    assert np.abs(got - expect).max() <= 1e-12 * np.abs(expect).max()
+
+
+def test_shear_wave_256_bitexact_and_viscosity(gpu):
+    # 256^2 periodic shear wave, u_x = 0.1 sin(2 pi y / 256), omega = 1.0, 1000 steps
+    from latticeboltzmann_amd.lattice import Lattice
+    n, steps, omega = 256, 1000, 1.0
+    f0 = o.shear_wave_init(n, n, 0.1, "x")
+    lat = Lattice(n, n, omega, "periodic", dtype=torch.float64).load(f0)
+    amps = [o.shear_wave_amplitude(f0, "x")]
+    ts = [0]
+    for chunk in range(10):
+        lat.step(steps // 10)
+        ts.append(lat.time)
+        amps.append(o.shear_wave_amplitude(lat.populations(), "x"))
+    expect = cref.run(f0, omega, steps, "periodic", 0.0)
+    assert np.array_equal(lat.populations(), expect)
+    nu = o.fit_viscosity(ts, amps, n)
+    assert abs(nu / o.viscosity_theory(omega) - 1) < 1e-3
+
+
+# ---------------------------------------------------------------------------
+# reductions
+# ---------------------------------------------------------------------------
+
+def test_sum_fp64_accurate_and_deterministic(gpu):
+    from latticeboltzmann_amd import kernels
+    x = RNG.random(3_000_001) * 10.0 ** RNG.uniform(-3, 3, 3_000_001)
+    t = dev(x)
+    s1 = float(kernels.total_sum(t).item())
+    s2 = float(kernels.total_sum(t).item())
+    exact = math.fsum(x)
+    assert s1 == s2
+    assert abs(s1 - exact) <= 1e-14 * abs(exact)
+    x32 = x.astype(np.float32)
+    s32 = float(kernels.total_sum(dev(x32)).item())
+    assert abs(s32 - math.fsum(x32.astype(np.float64))) <= 1e-13 * abs(exact)
+
+
+def test_velocity_residual(gpu):
+    from latticeboltzmann_amd import kernels
+    f = populations(40, 30)
+    uxp = dev(RNG.random((40, 30)) * 0.01)
+    uyp = dev(RNG.random((40, 30)) * 0.01)
+    _, ux, uy = o.moments(f)
+    expect = np.sqrt((ux - host(uxp)) ** 2 + (uy - host(uyp)) ** 2).max()
+    r = float(kernels.velocity_residual(dev(f), uxp, uyp).item())
+    assert abs(r - expect) <= 1e-15 * expect
+    assert np.array_equal(host(uxp), ux) and np.array_equal(host(uyp), uy)
+    assert float(kernels.velocity_residual(dev(f), uxp, uyp).item()) == 0.0
+
+
+# ---------------------------------------------------------------------------
+# full size (2048^2 cavity): size-independent properties
+# ---------------------------------------------------------------------------
+
+def test_cavity_2048_properties(gpu):
+    from latticeboltzmann_amd.lattice import Lattice, omega_for_cavity
+    n = 2048
+    omega = omega_for_cavity(n, 1000.0, 0.1)
+    assert abs(omega - 0.897343) < 1e-6
+    # without lid velocity the walls are a permutation and collide conserves
+    # mass: total mass stays n^2 to rounding
+    lat = Lattice(n, n, omega, "cavity", 0.0, torch.float64)
+    lat.step(50)
+    assert abs(lat.mass() - n * n) < 1e-9 * n * n
+    # with the lid: deterministic (bitwise) across runs and launch variants
+    a = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=0).step(40)
+    b = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=5).step(40)
+    assert torch.equal(a.f, b.f)
+    _, ux, _ = a.moments()
+    assert float(ux[:, -1].mean()) > 0.0
+    assert torch.isfinite(a.f).all()
