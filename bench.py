@@ -43,7 +43,8 @@ def parse():
                     "20000 steps of the 2048^2 cavity workload)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--variant", type=int, default=-1)
+    ap.add_argument("--variant", default="auto", help="launch shape of the step kernel "
+                    "(an integer, or 'auto' for the autotuner)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
     ap.add_argument("--cpu-budget", type=float, default=8.0)
@@ -125,7 +126,10 @@ def main():
     _native.load()
     n = args.n
     omega = omega_for_cavity(n, 1000.0, 0.1)
+    if args.variant != "auto":
+        args.variant = int(args.variant)
     lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant)
+    lat_variant = lat.variant
     sec = timed_steps(lat, args.steps, args.warmup)
     mlups = n * n * args.steps / sec / 1e6
     launch_s = sec / args.steps
@@ -134,21 +138,27 @@ def main():
     del lat
     torch.cuda.empty_cache()
 
-    extra = {"finite_state": ok, "omega": omega}
+    extra = {"finite_state": ok, "omega": omega, "variant": lat_variant}
     if not args.no_extra:
+        from latticeboltzmann_amd.precision import DeviationLattice
         big = 8192
-        lat = Lattice(big, big, omega_for_cavity(big, 1000.0, 0.1), "cavity", 0.1,
-                      torch.float64, variant=args.variant)
-        steps = 100
-        s = timed_steps(lat, steps, 10)
-        m = big * big * steps / s / 1e6
-        extra["cavity_8192_f64"] = {
-            "mlups": round(m, 1), "steps": steps,
-            "achieved_gbs": round(BYTES_PER_LUP["f64"] * m * 1e6 / 1e9, 1),
-            "frac_of_8tbs": round(BYTES_PER_LUP["f64"] * m * 1e6 / 1e9 / HBM_PEAK_GBS, 4),
-        }
-        del lat
-        torch.cuda.empty_cache()
+        om_big = omega_for_cavity(big, 1000.0, 0.1)
+        legs = {}
+        for name in ("f64", "f32", "bf16", "f16"):
+            if name == "f64":
+                lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant)
+            else:
+                lat = DeviationLattice(big, big, om_big, "cavity", 0.1, fmt=name, variant="auto")
+            steps = 200
+            s = timed_steps(lat, steps, 20)
+            m = big * big * steps / s / 1e6
+            gbs = BYTES_PER_LUP[name] * m * 1e6 / 1e9
+            legs[name] = {"mlups": round(m, 1), "steps": steps, "variant": lat.variant,
+                          "achieved_gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4)}
+            del lat
+            torch.cuda.empty_cache()
+        legs["f16_over_f64_mlups"] = round(legs["f16"]["mlups"] / legs["f64"]["mlups"], 3)
+        extra["cavity_8192"] = legs
         nbytes = 9 * big * big * 8
         extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
 

@@ -34,6 +34,13 @@ extern "C" {
 #define LBM_E_ALIAS (-4)     /* input and output buffers overlap              */
 #define LBM_E_WORKSPACE (-5) /* workspace too small                           */
 
+/* population layouts: element (i, k, l) lives at i*cs + k*rs + l
+ *   LBM_LAYOUT_SOA : f[i][k][l], cs = nx*ny, rs = ny   (the reference's f_ikl)
+ *   LBM_LAYOUT_ROWS: f[k][i][l], cs = ny, rs = 9*ny    (row-interleaved: the
+ *                    nine channel segments of an x-row are adjacent in HBM) */
+#define LBM_LAYOUT_SOA 0
+#define LBM_LAYOUT_ROWS 1
+
 /* boundary kinds for the fused step */
 #define LBM_BC_PERIODIC 0 /* periodic in x and y (shear wave)                 */
 #define LBM_BC_CAVITY 1   /* bounce-back walls, moving lid at l = ny-1        */
@@ -64,12 +71,13 @@ int lbm_collide_f64(double *f, int64_t n, double omega, double *rho, double *ux,
 int lbm_collide_f32(float *f, int64_t n, float omega, float *rho, float *ux,
                     float *uy, void *stream);
 
-/* Moments of f (9 x n) without modifying it, in the order of the numpy
- * collide (cavity_opt1.py:156-158); the reference computes them inline in its
- * drivers before each dump (cavity_opt2.py:280-281).                          */
-int lbm_moments_f64(const double *f, int64_t n, double *rho, double *ux, double *uy,
-                    void *stream);
-int lbm_moments_f32(const float *f, int64_t n, float *rho, float *ux, float *uy, void *stream);
+/* Moments of an nx x ny grid in either layout, f unchanged; rho/ux/uy are
+ * [k][l].  Order of the numpy collide (cavity_opt1.py:156-158); the reference
+ * computes them inline in its drivers before each dump (cavity_opt2.py:280-281). */
+int lbm_moments_f64(const double *f, int64_t nx, int64_t ny, int layout, double *rho,
+                    double *ux, double *uy, void *stream);
+int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *rho,
+                    float *ux, float *uy, void *stream);
 
 /* ---- streaming ------------------------------------------------------------------
  * Periodic pull streaming src -> dst (9 x nx x ny each, no overlap):
@@ -78,6 +86,11 @@ int lbm_moments_f32(const float *f, int64_t n, float *rho, float *ux, float *uy,
  * Replaces PyLB.stream (PyLB/Streaming.py:33-46).                              */
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
                         int elem_bytes, void *stream);
+
+/* Copy a population array between layouts (a permutation; src, dst disjoint).
+ * No reference counterpart: the reference has one layout.                      */
+int lbm_relayout(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,
+                 int src_layout, int dst_layout, void *stream);
 
 /* Streaming plus cavity walls: bounce-back at l = 0, k = 0, k = nx-1 and the
  * lid at l = ny-1 moving with u0 in +x, with the reference's override order.
@@ -103,6 +116,14 @@ int lbm_step_f64(const double *src, double *dst, int64_t nx, int64_t ny,
 int lbm_step_f32(const float *src, float *dst, int64_t nx, int64_t ny,
                  float omega, int bc, float u0, float *rho, float *ux, float *uy,
                  int variant, void *stream);
+/* The same step on either layout (lbm_step_* is layout LBM_LAYOUT_SOA); the
+ * arithmetic, and therefore every bit of the result, does not depend on it. */
+int lbm_step_layout_f64(const double *src, double *dst, int64_t nx, int64_t ny,
+                        int layout, double omega, int bc, double u0, double *rho,
+                        double *ux, double *uy, int variant, void *stream);
+int lbm_step_layout_f32(const float *src, float *dst, int64_t nx, int64_t ny,
+                        int layout, float omega, int bc, float u0, float *rho,
+                        float *ux, float *uy, int variant, void *stream);
 
 /* ---- reductions (FP64, deterministic) -------------------------------------------
  * Two-pass reductions with a fixed block count: identical inputs give
@@ -115,11 +136,49 @@ int lbm_sum_f64(const double *x, int64_t n, double *work, double *out, void *str
 /* sum of n floats accumulated in FP64 */
 int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stream);
 /* Velocity residual: out = max_j |(ux, uy)(f)_j - (ux_prev, uy_prev)_j| over
- * the nx*ny nodes of an FP64 population array f; ux_prev/uy_prev are then
- * overwritten with the current velocity.                                       */
-int lbm_velocity_residual_f64(const double *f, int64_t n, double *ux_prev,
-                              double *uy_prev, double *work, double *out,
-                              void *stream);
+ * the nx*ny nodes of an FP64 population array f in the given layout;
+ * ux_prev/uy_prev ([k][l]) are then overwritten with the current velocity.   */
+int lbm_velocity_residual_f64(const double *f, int64_t nx, int64_t ny, int layout,
+                              double *ux_prev, double *uy_prev, double *work,
+                              double *out, void *stream);
+
+/* ---- reduced-precision storage ---------------------------------------------------
+ * Populations stored as scaled deviations d_i = scale * (f_i - w_i) in FP32,
+ * BF16 or FP16 (fmt), in either layout, with the collide computed in FP32.
+ * FP64 is used only for encode/decode, the mass and the residual.
+ * No reference counterpart (the reference stores f itself, cavity_opt1.py:59). */
+#define LBM_FMT_F32 0
+#define LBM_FMT_BF16 1
+#define LBM_FMT_F16 2
+int lbm_dev_variant_count(void);
+/* FP64 populations f (reference layout, 9 x nx x ny) -> deviations d (layout) */
+int lbm_dev_encode(const double *f, void *d, int64_t nx, int64_t ny, int layout, int fmt,
+                   double scale, void *stream);
+/* deviations d (layout) -> FP64 populations f (reference layout) */
+int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout, int fmt,
+                   double scale, void *stream);
+/* Rounding of the 16-bit formats on store.  Both round the nine values of a
+ * node so that its mass and momentum carry a single rounding error (the
+ * momentum residual of the other channels moves into W/S, the mass residual
+ * into the rest population).  NEAREST rounds to nearest even; STOCHASTIC
+ * rounds up with probability equal to the distance from the value below,
+ * driven by a hash of (seed, node): unbiased, and deterministic for a given
+ * seed.  FP32 storage always rounds to nearest.                               */
+#define LBM_ROUND_NEAREST 0
+#define LBM_ROUND_STOCHASTIC 1
+/* One fused step src -> dst on stored deviations; ux/uy (nullable, FP32
+ * [k][l]) receive the pre-collision velocity.  variant in
+ * 0 .. lbm_dev_variant_count()-1 or -1.                                         */
+int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
+                 float omega, int bc, float u0, float scale, int rounding, uint32_t seed,
+                 float *ux, float *uy, int variant, void *stream);
+/* *out = sum_{i,k,l} f = nx*ny + (sum of stored values) / scale, in FP64     */
+int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,
+                 double *out, void *stream);
+/* velocity residual of stored deviations (velocity evaluated in FP64)        */
+int lbm_dev_velocity_residual(const void *d, int64_t nx, int64_t ny, int layout, int fmt,
+                              double scale, double *ux_prev, double *uy_prev, double *work,
+                              double *out, void *stream);
 
 #ifdef __cplusplus
 }

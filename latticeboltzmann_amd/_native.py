@@ -25,7 +25,10 @@ _ERRORS = {
 
 BC_PERIODIC = 0
 BC_CAVITY = 1
+LAYOUT_SOA = 0   # f[i][k][l], the reference's f_ikl
+LAYOUT_ROWS = 1  # f[k][i][l], row-interleaved
 FMT_F32, FMT_BF16, FMT_F16 = 0, 1, 2
+ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 
 _vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
 _f64, _f32 = ctypes.c_double, ctypes.c_float
@@ -38,17 +41,31 @@ SIGNATURES = {
     "lbm_equilibrium_f32": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "lbm_collide_f64": (_i32, [_vp, _i64, _f64, _vp, _vp, _vp, _vp]),
     "lbm_collide_f32": (_i32, [_vp, _i64, _f32, _vp, _vp, _vp, _vp]),
-    "lbm_moments_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
-    "lbm_moments_f32": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp]),
+    "lbm_moments_f64": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "lbm_moments_f32": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),
     "lbm_stream_periodic": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp]),
     "lbm_stream_bounce_back_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp]),
     "lbm_stream_bounce_back_f32": (_i32, [_vp, _vp, _i64, _i64, _f32, _vp]),
     "lbm_step_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _i32, _f64, _vp, _vp, _vp, _i32, _vp]),
     "lbm_step_f32": (_i32, [_vp, _vp, _i64, _i64, _f32, _i32, _f32, _vp, _vp, _vp, _i32, _vp]),
+    "lbm_step_layout_f64": (_i32, [_vp, _vp, _i64, _i64, _i32, _f64, _i32, _f64, _vp, _vp, _vp,
+                                   _i32, _vp]),
+    "lbm_step_layout_f32": (_i32, [_vp, _vp, _i64, _i64, _i32, _f32, _i32, _f32, _vp, _vp, _vp,
+                                   _i32, _vp]),
     "lbm_reduce_workspace_bytes": (_i64, []),
     "lbm_sum_f64": (_i32, [_vp, _i64, _vp, _vp, _vp]),
     "lbm_sum_f32": (_i32, [_vp, _i64, _vp, _vp, _vp]),
-    "lbm_velocity_residual_f64": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_velocity_residual_f64": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    # reduced-precision storage
+    "lbm_dev_variant_count": (_i32, []),
+    "lbm_dev_encode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
+    "lbm_dev_decode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
+    "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32, _i32,
+                            ctypes.c_uint32, _vp, _vp, _i32, _vp]),
+    "lbm_dev_mass": (_i32, [_vp, _i64, _i64, _i32, _f64, _vp, _vp, _vp]),
+    "lbm_dev_velocity_residual": (_i32, [_vp, _i64, _i64, _i32, _i32, _f64, _vp, _vp, _vp, _vp,
+                                         _vp]),
 }
 
 _lib = None

@@ -13,72 +13,29 @@
 // rows k-1, k, k+1 shifted by at most one element, so the reads stay
 // coalesced and every population is read exactly once per step.
 
+#include "d2q9_grid.hpp"
 #include "d2q9_node.hpp"
+#include "d2q9_reduce.hpp"
 #include "lbm_internal.hpp"
 
 #pragma clang fp contract(off)
 
 namespace lbm {
 
-template <bool NT, typename T>
-__device__ __forceinline__ T ld(const T *p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-template <bool NT, typename T>
-__device__ __forceinline__ void st(T *p, T v) {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
-}
-
-// Row base pointers of the nine pulls for row k.
-template <typename T>
-struct PullRows {
-    const T *p[9];
-    __device__ __forceinline__ PullRows(const T *src, int64_t plane, int k, int nx, int ny) {
-        const int km = k == 0 ? nx - 1 : k - 1;
-        const int kp = k == nx - 1 ? 0 : k + 1;
-        const int64_t rk = int64_t(k) * ny, rm = int64_t(km) * ny, rp = int64_t(kp) * ny;
-        p[0] = src + rk;
-        p[1] = src + plane + rm;          // E  pulls from (k-1, l)
-        p[2] = src + 2 * plane + rk;      // N  pulls from (k,   l-1)
-        p[3] = src + 3 * plane + rp;      // W  pulls from (k+1, l)
-        p[4] = src + 4 * plane + rk;      // S  pulls from (k,   l+1)
-        p[5] = src + 5 * plane + rm;      // NE pulls from (k-1, l-1)
-        p[6] = src + 6 * plane + rp;      // NW pulls from (k+1, l-1)
-        p[7] = src + 7 * plane + rp;      // SW pulls from (k+1, l+1)
-        p[8] = src + 8 * plane + rm;      // SE pulls from (k-1, l+1)
-    }
-};
-
-template <bool NT, typename T>
-__device__ __forceinline__ void pull9(const PullRows<T> &r, int l, int ny, T g[9]) {
-    const int lm = l == 0 ? ny - 1 : l - 1;
-    const int lp = l == ny - 1 ? 0 : l + 1;
-    g[0] = ld<NT>(r.p[0] + l);
-    g[1] = ld<NT>(r.p[1] + l);
-    g[2] = ld<NT>(r.p[2] + lm);
-    g[3] = ld<NT>(r.p[3] + l);
-    g[4] = ld<NT>(r.p[4] + lp);
-    g[5] = ld<NT>(r.p[5] + lm);
-    g[6] = ld<NT>(r.p[6] + lm);
-    g[7] = ld<NT>(r.p[7] + lp);
-    g[8] = ld<NT>(r.p[8] + lp);
-}
-
 // Cavity walls on the pulled values g of node (k, l), applied in the order of
 // cavity_opt1.py:201-225: bottom, lid (density from the post-bottom values),
-// left, right.  c points at the node's own pre-stream value of channel 0.
+// left, right.  c points at the node's own pre-stream value of channel 0;
+// channel i of the node is c[i * cs].
 template <typename T>
-__device__ __forceinline__ void cavity_walls(T g[9], const T *c, int64_t plane, int k, int l,
+__device__ __forceinline__ void cavity_walls(T g[9], const T *c, int64_t cs, int k, int l,
                                              int nx, int ny, T u0) {
     if (l == 0) {  // bottom: N <- S, NE <- SW, NW <- SE
-        g[2] = c[4 * plane];
-        g[5] = c[7 * plane];
-        g[6] = c[8 * plane];
+        g[2] = c[4 * cs];
+        g[5] = c[7 * cs];
+        g[6] = c[8 * cs];
     }
     if (l == ny - 1) {  // moving lid
-        const T c2 = c[2 * plane], c5 = c[5 * plane], c6 = c[6 * plane];
+        const T c2 = c[2 * cs], c5 = c[5 * cs], c6 = c[6 * cs];
         T rl = c6 + c2;
         rl = rl + c5;
         rl = rl + g[6];
@@ -93,52 +50,57 @@ __device__ __forceinline__ void cavity_walls(T g[9], const T *c, int64_t plane, 
         g[7] = c5 - t;
     }
     if (k == 0) {  // left: E <- W, NE <- SW, SE <- NW
-        g[1] = c[3 * plane];
-        g[5] = c[7 * plane];
-        g[8] = c[6 * plane];
+        g[1] = c[3 * cs];
+        g[5] = c[7 * cs];
+        g[8] = c[6 * cs];
     }
     if (k == nx - 1) {  // right: W <- E, NW <- SE, SW <- NE
-        g[3] = c[1 * plane];
-        g[6] = c[8 * plane];
-        g[7] = c[5 * plane];
+        g[3] = c[1 * cs];
+        g[6] = c[8 * cs];
+        g[7] = c[5 * cs];
     }
 }
 
 // ---------------------------------------------------------------------------
 // Fused step: pull (+ walls) + collide, src -> dst.
-// grid = nx * nseg blocks, nseg = ceil(ny / (V * kBlock)).
-template <typename T, int BC, bool MOM, int V, bool NT>
-__global__ __launch_bounds__(kBlock) void k_step(const T *__restrict__ src, T *__restrict__ dst,
-                                                 int nx, int ny, int64_t plane, int nseg, T omega,
-                                                 T u0, T *__restrict__ rho_o, T *__restrict__ ux_o,
-                                                 T *__restrict__ uy_o) {
+// grid = nx * nseg blocks of BS threads, nseg = ceil(ny / (V * BS)); block b
+// owns row k = b / nseg and V * BS consecutive y values of it.  NTL / NTS
+// mark the loads / stores non-temporal (each population is touched once per
+// step, so there is nothing to keep in cache beyond the one-element overlap
+// of neighbouring pulls).
+template <typename T, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
+__global__ __launch_bounds__(BS) void k_step(const T *__restrict__ src, T *__restrict__ dst,
+                                             int nx, int ny, Layout L, int nseg, T omega, T u0,
+                                             T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                             T *__restrict__ uy_o) {
     const int b = blockIdx.x;
     const int k = b / nseg;
     const int seg = b - k * nseg;
-    const PullRows<T> rows(src, plane, k, nx, ny);
-    const int64_t rk = int64_t(k) * ny;
-    const int l0 = seg * (V * kBlock) + threadIdx.x;
+    const PullRows<T> rows(src, L, k, nx);
+    const int64_t rk = int64_t(k) * L.rs;
+    const int l0 = seg * (V * BS) + threadIdx.x;
 
     T g[V][9];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-        const int l = l0 + v * kBlock;
-        if (l < ny) pull9<NT>(rows, l, ny, g[v]);
+        const int l = l0 + v * BS;
+        if (l < ny) pull9<NTL>(rows, l, ny, g[v]);
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-        const int l = l0 + v * kBlock;
+        const int l = l0 + v * BS;
         if (l >= ny) continue;
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g[v], src + rk + l, plane, k, l, nx, ny, u0);
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g[v], src + rk + l, L.cs, k, l, nx, ny, u0);
         T rho, ux, uy;
         collide_opt1(g[v], omega, rho, ux, uy);
         T *d = dst + rk + l;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) st<NT>(d + i * plane, g[v][i]);
+        for (int i = 0; i < 9; ++i) st<NTS>(d + i * L.cs, g[v][i]);
         if constexpr (MOM) {
-            rho_o[rk + l] = rho;
-            ux_o[rk + l] = ux;
-            uy_o[rk + l] = uy;
+            const int64_t j = int64_t(k) * ny + l;
+            rho_o[j] = rho;
+            ux_o[j] = ux;
+            uy_o[j] = uy;
         }
     }
 }
@@ -147,35 +109,71 @@ __global__ __launch_bounds__(kBlock) void k_step(const T *__restrict__ src, T *_
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stream_bounce(const T *__restrict__ src,
                                                           T *__restrict__ dst, int nx, int ny,
-                                                          int64_t plane, int nseg, T u0) {
+                                                          Layout L, int nseg, T u0) {
     const int b = blockIdx.x;
     const int k = b / nseg;
     const int l = (b - k * nseg) * kBlock + threadIdx.x;
     if (l >= ny) return;
-    const PullRows<T> rows(src, plane, k, nx, ny);
-    const int64_t rk = int64_t(k) * ny;
+    const PullRows<T> rows(src, L, k, nx);
+    const int64_t rk = int64_t(k) * L.rs;
     T g[9];
     pull9<false>(rows, l, ny, g);
-    cavity_walls(g, src + rk + l, plane, k, l, nx, ny, u0);
+    cavity_walls(g, src + rk + l, L.cs, k, l, nx, ny, u0);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dst[i * plane + rk + l] = g[i];
+    for (int i = 0; i < 9; ++i) dst[i * L.cs + rk + l] = g[i];
 }
 
 // Periodic streaming of U-sized elements (a pure permutation, PyLB.stream).
 template <typename U>
 __global__ __launch_bounds__(kBlock) void k_stream_periodic(const U *__restrict__ src,
                                                             U *__restrict__ dst, int nx, int ny,
-                                                            int64_t plane, int nseg) {
+                                                            Layout L, int nseg) {
     const int b = blockIdx.x;
     const int k = b / nseg;
     const int l = (b - k * nseg) * kBlock + threadIdx.x;
     if (l >= ny) return;
-    const PullRows<U> rows(src, plane, k, nx, ny);
+    const PullRows<U> rows(src, L, k, nx);
     U g[9];
     pull9<false>(rows, l, ny, g);
-    const int64_t rk = int64_t(k) * ny;
+    const int64_t rk = int64_t(k) * L.rs;
 #pragma unroll
-    for (int i = 0; i < 9; ++i) dst[i * plane + rk + l] = g[i];
+    for (int i = 0; i < 9; ++i) dst[i * L.cs + rk + l] = g[i];
+}
+
+// Copy between layouts: grid = 9 * nx * nseg blocks.
+template <typename U>
+__global__ __launch_bounds__(kBlock) void k_relayout(const U *__restrict__ src,
+                                                     U *__restrict__ dst, int nx, int ny,
+                                                     Layout Ls, Layout Ld, int nseg) {
+    const int64_t b = blockIdx.x;
+    const int64_t row = b / nseg;  // row = i * nx + k
+    const int l = int(b - row * nseg) * kBlock + threadIdx.x;
+    if (l >= ny) return;
+    const int64_t i = row / nx, k = row - i * nx;
+    dst[i * Ld.cs + k * Ld.rs + l] = src[i * Ls.cs + k * Ls.rs + l];
+}
+
+// Moments of a grid in either layout; scalar fields are [k][l].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_moments_grid(const T *__restrict__ f, int nx, int ny,
+                                                         Layout L, int nseg,
+                                                         T *__restrict__ rho_o,
+                                                         T *__restrict__ ux_o,
+                                                         T *__restrict__ uy_o) {
+    const int b = blockIdx.x;
+    const int k = b / nseg;
+    const int l = (b - k * nseg) * kBlock + threadIdx.x;
+    if (l >= ny) return;
+    const T *c = f + int64_t(k) * L.rs + l;
+    T g[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) g[i] = c[i * L.cs];
+    T rho, ux, uy;
+    moments_opt1(g, rho, ux, uy);
+    const int64_t j = int64_t(k) * ny + l;
+    rho_o[j] = rho;
+    ux_o[j] = ux;
+    uy_o[j] = uy;
 }
 
 // Collide in place on f[9][n] (PyLB.collide).
@@ -199,23 +197,6 @@ __global__ __launch_bounds__(kBlock) void k_collide(T *__restrict__ f, int64_t n
     }
 }
 
-// Moments of f[9][n] in the opt1 order (cavity_opt1.py:156-158).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_moments(const T *__restrict__ f, int64_t n,
-                                                    T *__restrict__ rho_o, T *__restrict__ ux_o,
-                                                    T *__restrict__ uy_o) {
-    const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
-    if (j >= n) return;
-    T g[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) g[i] = f[i * n + j];
-    T rho, ux, uy;
-    moments_opt1(g, rho, ux, uy);
-    rho_o[j] = rho;
-    ux_o[j] = ux;
-    uy_o[j] = uy;
-}
-
 // f[9][n] = f_eq(rho, ux, uy) (PyLB.equilibrium).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_equilibrium(const T *__restrict__ rho,
@@ -230,42 +211,6 @@ __global__ __launch_bounds__(kBlock) void k_equilibrium(const T *__restrict__ rh
     for (int i = 0; i < 9; ++i) f[i * n + j] = feq[i];
 }
 
-// ---------------------------------------------------------------------------
-// Deterministic reductions: pass 1 with a fixed grid of kReduceBlocks blocks,
-// each thread accumulating a grid-strided sequence with Neumaier compensation,
-// then a fixed-shape tree in LDS; pass 2 folds the partials in one block.
-
-__device__ __forceinline__ void neumaier_add(double &s, double &c, double x) {
-    const double t = s + x;
-    if (fabs(s) >= fabs(x)) c += (s - t) + x;
-    else c += (x - t) + s;
-    s = t;
-}
-
-__device__ __forceinline__ double block_sum(double v) {
-    __shared__ double red[kBlock];
-    red[threadIdx.x] = v;
-    __syncthreads();
-#pragma unroll
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = red[threadIdx.x] + red[threadIdx.x + w];
-        __syncthreads();
-    }
-    return red[0];
-}
-
-__device__ __forceinline__ double block_max(double v) {
-    __shared__ double red[kBlock];
-    red[threadIdx.x] = v;
-    __syncthreads();
-#pragma unroll
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
-        __syncthreads();
-    }
-    return red[0];
-}
-
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_sum_pass1(const T *__restrict__ x, int64_t n,
                                                       double *__restrict__ partial) {
@@ -277,12 +222,13 @@ __global__ __launch_bounds__(kBlock) void k_sum_pass1(const T *__restrict__ x, i
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
+// out = bias + sum of the np partials
 __global__ __launch_bounds__(kBlock) void k_sum_pass2(const double *__restrict__ partial, int np,
-                                                      double *__restrict__ out) {
+                                                      double bias, double *__restrict__ out) {
     double s = 0.0, c = 0.0;
     for (int j = threadIdx.x; j < np; j += kBlock) neumaier_add(s, c, partial[j]);
     const double tot = block_sum(s + c);
-    if (threadIdx.x == 0) out[0] = tot;
+    if (threadIdx.x == 0) out[0] = bias + tot;
 }
 
 __global__ __launch_bounds__(kBlock) void k_max_pass2(const double *__restrict__ partial, int np,
@@ -293,16 +239,20 @@ __global__ __launch_bounds__(kBlock) void k_max_pass2(const double *__restrict__
     if (threadIdx.x == 0) out[0] = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restrict__ f, int64_t n,
+__global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restrict__ f, int nx,
+                                                           int ny, Layout L,
                                                            double *__restrict__ uxp,
                                                            double *__restrict__ uyp,
                                                            double *__restrict__ partial) {
     double m = 0.0;
+    const int64_t n = int64_t(nx) * ny;
     const int64_t stride = int64_t(gridDim.x) * kBlock;
     for (int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride) {
+        const int64_t k = j / ny, l = j - k * ny;
+        const double *c = f + k * L.rs + l;
         double g[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = f[i * n + j];
+        for (int i = 0; i < 9; ++i) g[i] = c[i * L.cs];
         double rho, ux, uy;
         moments_opt1(g, rho, ux, uy);
         const double dx = ux - uxp[j], dy = uy - uyp[j];
@@ -317,60 +267,77 @@ __global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restr
 // ---------------------------------------------------------------------------
 // Launchers
 
-struct StepShape {
-    int v;
-    bool nt;
-};
-// Launch shapes of the fused step, indexed by `variant`.
-constexpr StepShape kStepShapes[] = {{1, false}, {2, false}, {4, false},
-                                     {1, true},  {2, true},  {4, true}};
-constexpr int kNumStepShapes = int(sizeof(kStepShapes) / sizeof(kStepShapes[0]));
-constexpr int kDefaultStepShape = 1;
+// Launch shapes of the fused step, indexed by `variant`:
+// (threads per block, nodes per thread, non-temporal loads, non-temporal stores).
+#define LBM_STEP_SHAPES(X)        \
+    X(0, 256, 1, false, false)    \
+    X(1, 256, 2, false, false)    \
+    X(2, 256, 4, false, false)    \
+    X(3, 256, 1, false, true)     \
+    X(4, 256, 2, false, true)     \
+    X(5, 256, 4, false, true)     \
+    X(6, 512, 1, false, true)     \
+    X(7, 512, 2, false, true)     \
+    X(8, 128, 4, false, true)     \
+    X(9, 256, 1, true, true)      \
+    X(10, 512, 1, true, true)     \
+    X(11, 1024, 1, false, true)
 
-template <typename T, int BC, bool MOM, int V, bool NT>
-void launch_step_v(const T *src, T *dst, int nx, int ny, T omega, T u0, T *rho, T *ux, T *uy,
-                   hipStream_t s) {
-    const int nseg = (ny + V * kBlock - 1) / (V * kBlock);
-    const int64_t plane = int64_t(nx) * ny;
-    hipLaunchKernelGGL((k_step<T, BC, MOM, V, NT>), dim3(unsigned(int64_t(nx) * nseg)),
-                       dim3(kBlock), 0, s, src, dst, nx, ny, plane, nseg, omega, u0, rho, ux, uy);
+#define LBM_COUNT_SHAPE(id, bs, v, ntl, nts) +1
+constexpr int kNumStepShapes = 0 LBM_STEP_SHAPES(LBM_COUNT_SHAPE);
+constexpr int kDefaultStepShape = 6;
+
+// Blocks of kBlock threads covering ny (one thread per node) times nx rows.
+inline int64_t row_blocks(int64_t nx, int64_t ny, int per_thread = 1) {
+    const int64_t span = int64_t(per_thread) * kBlock;
+    return nx * ((ny + span - 1) / span);
+}
+
+template <typename T, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
+void launch_step_v(const T *src, T *dst, int nx, int ny, Layout L, T omega, T u0, T *rho, T *ux,
+                   T *uy, hipStream_t s) {
+    const int nseg = (ny + V * BS - 1) / (V * BS);
+    hipLaunchKernelGGL((k_step<T, BC, MOM, BS, V, NTL, NTS>), dim3(unsigned(int64_t(nx) * nseg)),
+                       dim3(BS), 0, s, src, dst, nx, ny, L, nseg, omega, u0, rho, ux, uy);
 }
 
 template <typename T, int BC, bool MOM>
-void launch_step_bc(const T *src, T *dst, int nx, int ny, T omega, T u0, T *rho, T *ux, T *uy,
-                    int shape, hipStream_t s) {
+void launch_step_bc(const T *src, T *dst, int nx, int ny, Layout L, T omega, T u0, T *rho, T *ux,
+                    T *uy, int shape, hipStream_t s) {
+#define LBM_LAUNCH_SHAPE(id, bs, v, ntl, nts)                                                  \
+    case id:                                                                                  \
+        launch_step_v<T, BC, MOM, bs, v, ntl, nts>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s); \
+        break;
     switch (shape) {
-        case 0: launch_step_v<T, BC, MOM, 1, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
-        case 1: launch_step_v<T, BC, MOM, 2, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
-        case 2: launch_step_v<T, BC, MOM, 4, false>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
-        case 3: launch_step_v<T, BC, MOM, 1, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
-        case 4: launch_step_v<T, BC, MOM, 2, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
-        default: launch_step_v<T, BC, MOM, 4, true>(src, dst, nx, ny, omega, u0, rho, ux, uy, s); break;
+        LBM_STEP_SHAPES(LBM_LAUNCH_SHAPE)
+        default: break;
     }
+#undef LBM_LAUNCH_SHAPE
 }
 
 template <typename T>
-int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, T omega, int bc, T u0, T *rho, T *ux,
-              T *uy, int variant, void *stream) {
+int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega, int bc, T u0,
+              T *rho, T *ux, T *uy, int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (!valid_layout(layout)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
     if (variant < -1 || variant >= kNumStepShapes) return LBM_E_ARG;
     const bool mom = rho != nullptr;
     if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
     const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int64_t nseg_max = (ny + kBlock - 1) / kBlock;
-    if (nx * nseg_max > 0x7fffffffLL) return LBM_E_SIZE;
+    if (row_blocks(nx, ny) > 0x7fffffffLL) return LBM_E_SIZE;
     const int shape = variant < 0 ? kDefaultStepShape : variant;
     hipStream_t s = as_stream(stream);
     const int inx = int(nx), iny = int(ny);
+    const Layout L = make_layout(layout, nx, ny);
     if (bc == LBM_BC_CAVITY) {
-        if (mom) launch_step_bc<T, LBM_BC_CAVITY, true>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
-        else launch_step_bc<T, LBM_BC_CAVITY, false>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+        if (mom) launch_step_bc<T, LBM_BC_CAVITY, true>(src, dst, inx, iny, L, omega, u0, rho, ux, uy, shape, s);
+        else launch_step_bc<T, LBM_BC_CAVITY, false>(src, dst, inx, iny, L, omega, u0, rho, ux, uy, shape, s);
     } else {
-        if (mom) launch_step_bc<T, LBM_BC_PERIODIC, true>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
-        else launch_step_bc<T, LBM_BC_PERIODIC, false>(src, dst, inx, iny, omega, u0, rho, ux, uy, shape, s);
+        if (mom) launch_step_bc<T, LBM_BC_PERIODIC, true>(src, dst, inx, iny, L, omega, u0, rho, ux, uy, shape, s);
+        else launch_step_bc<T, LBM_BC_PERIODIC, false>(src, dst, inx, iny, L, omega, u0, rho, ux, uy, shape, s);
     }
     return launch_status();
 }
@@ -381,19 +348,29 @@ int stream_bounce_impl(const T *src, T *dst, int64_t nx, int64_t ny, T u0, void 
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
     const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int64_t nseg = (ny + kBlock - 1) / kBlock;
-    if (nx * nseg > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL(k_stream_bounce<T>, dim3(unsigned(nx * nseg)), dim3(kBlock), 0,
-                       as_stream(stream), src, dst, int(nx), int(ny), nx * ny, int(nseg), u0);
+    const int64_t nb = row_blocks(nx, ny);
+    if (nb > 0x7fffffffLL) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_stream_bounce<T>, dim3(unsigned(nb)), dim3(kBlock), 0, as_stream(stream),
+                       src, dst, int(nx), int(ny), make_layout(LBM_LAYOUT_SOA, nx, ny),
+                       int(nb / nx), u0);
     return launch_status();
 }
 
 template <typename U>
-void launch_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, int64_t nseg,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_stream_periodic<U>, dim3(unsigned(nx * nseg)), dim3(kBlock), 0, s,
+void launch_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, hipStream_t s) {
+    const int64_t nb = row_blocks(nx, ny);
+    hipLaunchKernelGGL(k_stream_periodic<U>, dim3(unsigned(nb)), dim3(kBlock), 0, s,
                        static_cast<const U *>(src), static_cast<U *>(dst), int(nx), int(ny),
-                       nx * ny, int(nseg));
+                       make_layout(LBM_LAYOUT_SOA, nx, ny), int(nb / nx));
+}
+
+template <typename U>
+void launch_relayout(const void *src, void *dst, int64_t nx, int64_t ny, Layout Ls, Layout Ld,
+                     hipStream_t s) {
+    const int64_t nseg = (ny + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_relayout<U>, dim3(unsigned(9 * nx * nseg)), dim3(kBlock), 0, s,
+                       static_cast<const U *>(src), static_cast<U *>(dst), int(nx), int(ny), Ls,
+                       Ld, int(nseg));
 }
 
 template <typename T>
@@ -414,13 +391,16 @@ int collide_impl(T *f, int64_t n, T omega, T *rho, T *ux, T *uy, void *stream) {
 }
 
 template <typename T>
-int moments_impl(const T *f, int64_t n, T *rho, T *ux, T *uy, void *stream) {
+int moments_impl(const T *f, int64_t nx, int64_t ny, int layout, T *rho, T *ux, T *uy,
+                 void *stream) {
     if (!f || !rho || !ux || !uy) return LBM_E_NULL;
-    if (n <= 0 || n > (int64_t(1) << 40)) return LBM_E_SIZE;
-    const int64_t nb = (n + kBlock - 1) / kBlock;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (!valid_layout(layout)) return LBM_E_ARG;
+    const int64_t nb = row_blocks(nx, ny);
     if (nb > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL(k_moments<T>, dim3(unsigned(nb)), dim3(kBlock), 0, as_stream(stream), f, n,
-                       rho, ux, uy);
+    hipLaunchKernelGGL(k_moments_grid<T>, dim3(unsigned(nb)), dim3(kBlock), 0, as_stream(stream),
+                       f, int(nx), int(ny), make_layout(layout, nx, ny), int(nb / nx), rho, ux,
+                       uy);
     return launch_status();
 }
 
@@ -438,13 +418,20 @@ int equilibrium_impl(const T *rho, const T *ux, const T *uy, T *f, int64_t n, vo
     return launch_status();
 }
 
+void launch_sum_pass2(const double *partial, int np, double bias, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_sum_pass2, dim3(1), dim3(kBlock), 0, s, partial, np, bias, out);
+}
+void launch_max_pass2(const double *partial, int np, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_max_pass2, dim3(1), dim3(kBlock), 0, s, partial, np, out);
+}
+
 template <typename T>
 int sum_impl(const T *x, int64_t n, double *work, double *out, void *stream) {
     if (!x || !work || !out) return LBM_E_NULL;
     if (n <= 0) return LBM_E_SIZE;
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_sum_pass1<T>, dim3(kReduceBlocks), dim3(kBlock), 0, s, x, n, work);
-    hipLaunchKernelGGL(k_sum_pass2, dim3(1), dim3(kBlock), 0, s, work, kReduceBlocks, out);
+    launch_sum_pass2(work, kReduceBlocks, 0.0, out, s);
     return launch_status();
 }
 
@@ -475,12 +462,13 @@ int lbm_collide_f32(float *f, int64_t n, float omega, float *rho, float *ux, flo
     return collide_impl(f, n, omega, rho, ux, uy, stream);
 }
 
-int lbm_moments_f64(const double *f, int64_t n, double *rho, double *ux, double *uy,
-                    void *stream) {
-    return moments_impl(f, n, rho, ux, uy, stream);
+int lbm_moments_f64(const double *f, int64_t nx, int64_t ny, int layout, double *rho, double *ux,
+                    double *uy, void *stream) {
+    return moments_impl(f, nx, ny, layout, rho, ux, uy, stream);
 }
-int lbm_moments_f32(const float *f, int64_t n, float *rho, float *ux, float *uy, void *stream) {
-    return moments_impl(f, n, rho, ux, uy, stream);
+int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *rho, float *ux,
+                    float *uy, void *stream) {
+    return moments_impl(f, nx, ny, layout, rho, ux, uy, stream);
 }
 
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,
@@ -491,14 +479,34 @@ int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny, int 
         return LBM_E_ARG;
     const int64_t bytes = 9 * nx * ny * elem_bytes;
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int64_t nseg = (ny + kBlock - 1) / kBlock;
-    if (nx * nseg > 0x7fffffffLL) return LBM_E_SIZE;
+    if (row_blocks(nx, ny) > 0x7fffffffLL) return LBM_E_SIZE;
     hipStream_t s = as_stream(stream);
     switch (elem_bytes) {
-        case 1: launch_stream_periodic<uint8_t>(src, dst, nx, ny, nseg, s); break;
-        case 2: launch_stream_periodic<uint16_t>(src, dst, nx, ny, nseg, s); break;
-        case 4: launch_stream_periodic<uint32_t>(src, dst, nx, ny, nseg, s); break;
-        default: launch_stream_periodic<uint64_t>(src, dst, nx, ny, nseg, s); break;
+        case 1: launch_stream_periodic<uint8_t>(src, dst, nx, ny, s); break;
+        case 2: launch_stream_periodic<uint16_t>(src, dst, nx, ny, s); break;
+        case 4: launch_stream_periodic<uint32_t>(src, dst, nx, ny, s); break;
+        default: launch_stream_periodic<uint64_t>(src, dst, nx, ny, s); break;
+    }
+    return launch_status();
+}
+
+int lbm_relayout(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,
+                 int src_layout, int dst_layout, void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (!valid_layout(src_layout) || !valid_layout(dst_layout)) return LBM_E_ARG;
+    if (elem_bytes != 1 && elem_bytes != 2 && elem_bytes != 4 && elem_bytes != 8)
+        return LBM_E_ARG;
+    const int64_t bytes = 9 * nx * ny * elem_bytes;
+    if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
+    if (9 * row_blocks(nx, ny) > 0x7fffffffLL) return LBM_E_SIZE;
+    const Layout Ls = make_layout(src_layout, nx, ny), Ld = make_layout(dst_layout, nx, ny);
+    hipStream_t s = as_stream(stream);
+    switch (elem_bytes) {
+        case 1: launch_relayout<uint8_t>(src, dst, nx, ny, Ls, Ld, s); break;
+        case 2: launch_relayout<uint16_t>(src, dst, nx, ny, Ls, Ld, s); break;
+        case 4: launch_relayout<uint32_t>(src, dst, nx, ny, Ls, Ld, s); break;
+        default: launch_relayout<uint64_t>(src, dst, nx, ny, Ls, Ld, s); break;
     }
     return launch_status();
 }
@@ -514,11 +522,21 @@ int lbm_stream_bounce_back_f32(const float *src, float *dst, int64_t nx, int64_t
 
 int lbm_step_f64(const double *src, double *dst, int64_t nx, int64_t ny, double omega, int bc,
                  double u0, double *rho, double *ux, double *uy, int variant, void *stream) {
-    return step_impl(src, dst, nx, ny, omega, bc, u0, rho, ux, uy, variant, stream);
+    return step_impl(src, dst, nx, ny, LBM_LAYOUT_SOA, omega, bc, u0, rho, ux, uy, variant, stream);
 }
 int lbm_step_f32(const float *src, float *dst, int64_t nx, int64_t ny, float omega, int bc,
                  float u0, float *rho, float *ux, float *uy, int variant, void *stream) {
-    return step_impl(src, dst, nx, ny, omega, bc, u0, rho, ux, uy, variant, stream);
+    return step_impl(src, dst, nx, ny, LBM_LAYOUT_SOA, omega, bc, u0, rho, ux, uy, variant, stream);
+}
+int lbm_step_layout_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout,
+                        double omega, int bc, double u0, double *rho, double *ux, double *uy,
+                        int variant, void *stream) {
+    return step_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
+}
+int lbm_step_layout_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout,
+                        float omega, int bc, float u0, float *rho, float *ux, float *uy,
+                        int variant, void *stream) {
+    return step_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
 }
 
 int64_t lbm_reduce_workspace_bytes(void) { return int64_t(kReduceBlocks) * int64_t(sizeof(double)); }
@@ -530,13 +548,15 @@ int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stre
     return sum_impl(x, n, work, out, stream);
 }
 
-int lbm_velocity_residual_f64(const double *f, int64_t n, double *ux_prev, double *uy_prev,
-                              double *work, double *out, void *stream) {
+int lbm_velocity_residual_f64(const double *f, int64_t nx, int64_t ny, int layout,
+                              double *ux_prev, double *uy_prev, double *work, double *out,
+                              void *stream) {
     if (!f || !ux_prev || !uy_prev || !work || !out) return LBM_E_NULL;
-    if (n <= 0) return LBM_E_SIZE;
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (!valid_layout(layout)) return LBM_E_ARG;
     hipStream_t s = as_stream(stream);
-    hipLaunchKernelGGL(k_residual_pass1, dim3(kReduceBlocks), dim3(kBlock), 0, s, f, n, ux_prev,
-                       uy_prev, work);
+    hipLaunchKernelGGL(k_residual_pass1, dim3(kReduceBlocks), dim3(kBlock), 0, s, f, int(nx),
+                       int(ny), make_layout(layout, nx, ny), ux_prev, uy_prev, work);
     hipLaunchKernelGGL(k_max_pass2, dim3(1), dim3(kBlock), 0, s, work, kReduceBlocks, out);
     return launch_status();
 }

@@ -25,6 +25,15 @@ inline bool overlaps(const void *a, int64_t abytes, const void *b, int64_t bbyte
     return pa < pb + bbytes && pb < pa + abytes;
 }
 
+// Second passes of the two-pass reductions (defined in d2q9_kernels.hip):
+// fold np partials into *out with one block (sum: *out = bias + sum).
+void launch_sum_pass2(const double *partial, int np, double bias, double *out, hipStream_t s);
+void launch_max_pass2(const double *partial, int np, double *out, hipStream_t s);
+
+inline bool valid_layout(int layout) {
+    return layout == LBM_LAYOUT_SOA || layout == LBM_LAYOUT_ROWS;
+}
+
 inline bool valid_grid(int64_t nx, int64_t ny) {
     if (nx <= 0 || ny <= 0 || nx > kMaxSide || ny > kMaxSide) return false;
     // total element count must stay far from int64 overflow

@@ -2,8 +2,12 @@
 
 Each function validates shapes, dtypes, devices and contiguity on the host,
 then calls the C ABI (``include/lbm_d2q9.h``) on torch's current stream.
-Population tensors have shape ``(9, nx, ny)`` (or ``(9, n)`` where only the
-node count matters), C-contiguous, float32 or float64, on a ROCm device.
+Population tensors are C-contiguous, float32 or float64, on a ROCm device,
+in one of two layouts:
+
+* ``LAYOUT_SOA``: shape ``(9, nx, ny)``, the reference's ``f_ikl``;
+* ``LAYOUT_ROWS``: shape ``(nx, 9, ny)``, the nine channel segments of each
+  x-row adjacent (the layout the device-resident solver keeps in HBM).
 """
 
 import torch
@@ -30,11 +34,20 @@ def _require_device(*tensors):
             raise TypeError("tensor must be C-contiguous")
 
 
-def _populations(f, what):
+def _populations(f, what, layout=nat.LAYOUT_SOA):
+    """(nx, ny) of a population tensor in ``layout``."""
     _require_device(f)
-    if f.dim() != 3 or f.shape[0] != 9:
-        raise ValueError("%s: populations must have shape (9, nx, ny), got %s" % (what, tuple(f.shape)))
-    return f.shape[1], f.shape[2]
+    if layout == nat.LAYOUT_SOA:
+        if f.dim() != 3 or f.shape[0] != 9:
+            raise ValueError("%s: populations must have shape (9, nx, ny), got %s"
+                             % (what, tuple(f.shape)))
+        return f.shape[1], f.shape[2]
+    if layout == nat.LAYOUT_ROWS:
+        if f.dim() != 3 or f.shape[1] != 9:
+            raise ValueError("%s: row-interleaved populations must have shape (nx, 9, ny), got %s"
+                             % (what, tuple(f.shape)))
+        return f.shape[0], f.shape[2]
+    raise ValueError("%s: unknown layout %r" % (what, layout))
 
 
 def _same(a, b, what):
@@ -79,16 +92,28 @@ def collide_(f, omega, moments=False):
     return (rho, ux, uy) if moments else None
 
 
-def moments(f):
-    """(rho, ux, uy) of populations ``f`` (shape ``(9, ...)``), f unchanged."""
-    _require_device(f)
+def moments(f, layout=nat.LAYOUT_SOA):
+    """(rho, ux, uy) of populations ``f``, each ``(nx, ny)``; f unchanged."""
+    nx, ny = _populations(f, "moments", layout)
     suf = _suffix(f, "moments")
-    if f.dim() < 2 or f.shape[0] != 9:
-        raise ValueError("moments: populations must have shape (9, ...)")
-    rho, ux, uy = (torch.empty(f.shape[1:], dtype=f.dtype, device=f.device) for _ in range(3))
-    nat.call("lbm_moments_" + suf, nat.ptr(f), f.numel() // 9, nat.ptr(rho), nat.ptr(ux),
+    rho, ux, uy = (torch.empty((nx, ny), dtype=f.dtype, device=f.device) for _ in range(3))
+    nat.call("lbm_moments_" + suf, nat.ptr(f), nx, ny, int(layout), nat.ptr(rho), nat.ptr(ux),
              nat.ptr(uy), nat.stream_ptr())
     return rho, ux, uy
+
+
+def relayout(f, src_layout, dst_layout, out=None):
+    """Copy populations between layouts (bit-exact permutation)."""
+    nx, ny = _populations(f, "relayout", src_layout)
+    shape = (9, nx, ny) if dst_layout == nat.LAYOUT_SOA else (nx, 9, ny)
+    if out is None:
+        out = torch.empty(shape, dtype=f.dtype, device=f.device)
+    _require_device(out)
+    if tuple(out.shape) != shape or out.dtype != f.dtype:
+        raise ValueError("relayout: output must have shape %s and the input's dtype" % (shape,))
+    nat.call("lbm_relayout", nat.ptr(f), nat.ptr(out), nx, ny, f.element_size(),
+             int(src_layout), int(dst_layout), nat.stream_ptr())
+    return out
 
 
 def stream(f, out=None):
@@ -120,13 +145,14 @@ def stream_bounce_back(f, u0=0.1, out=None):
     return out
 
 
-def step(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1):
+def step(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
+         layout=nat.LAYOUT_SOA):
     """One fused step ``src -> dst`` (pull streaming, walls if cavity, collide).
 
     ``moments`` may be a tuple ``(rho, ux, uy)`` of ``(nx, ny)`` tensors that
     receive the pre-collision moments.
     """
-    nx, ny = _populations(src, "step")
+    nx, ny = _populations(src, "step", layout)
     _require_device(dst)
     suf = _suffix(src, "step")
     _same(src, dst, "step")
@@ -140,8 +166,9 @@ def step(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1):
             _same(src, t, "step")
             if t.shape != (nx, ny):
                 raise ValueError("step: moment arrays must have shape (nx, ny)")
-    nat.call("lbm_step_" + suf, nat.ptr(src), nat.ptr(dst), nx, ny, float(omega), int(bc),
-             float(u0), nat.ptr(rho), nat.ptr(ux), nat.ptr(uy), int(variant), nat.stream_ptr())
+    nat.call("lbm_step_layout_" + suf, nat.ptr(src), nat.ptr(dst), nx, ny, int(layout),
+             float(omega), int(bc), float(u0), nat.ptr(rho), nat.ptr(ux), nat.ptr(uy),
+             int(variant), nat.stream_ptr())
     return dst
 
 
@@ -174,16 +201,19 @@ def total_sum(x, ws=None):
     return ws.out.clone()
 
 
-def velocity_residual(f, ux_prev, uy_prev, ws=None):
+def velocity_residual(f, ux_prev, uy_prev, ws=None, layout=nat.LAYOUT_SOA):
     """max_j |u_j(f) - u_prev_j| for FP64 populations; u_prev is updated.
 
     Returns a device scalar (1-element FP64 tensor).
     """
-    nx, ny = _populations(f, "velocity_residual")
+    nx, ny = _populations(f, "velocity_residual", layout)
     if f.dtype != torch.float64:
         raise TypeError("velocity_residual: FP64 populations expected")
     _require_device(ux_prev, uy_prev)
+    for t in (ux_prev, uy_prev):
+        if t.dtype != torch.float64 or tuple(t.shape) != (nx, ny):
+            raise ValueError("velocity_residual: previous velocity must be FP64 (nx, ny)")
     ws = ws or _workspace(f.device)
-    nat.call("lbm_velocity_residual_f64", nat.ptr(f), nx * ny, nat.ptr(ux_prev), nat.ptr(uy_prev),
-             nat.ptr(ws.work), nat.ptr(ws.out), nat.stream_ptr())
+    nat.call("lbm_velocity_residual_f64", nat.ptr(f), nx, ny, int(layout), nat.ptr(ux_prev),
+             nat.ptr(uy_prev), nat.ptr(ws.work), nat.ptr(ws.out), nat.stream_ptr())
     return ws.out.clone()

@@ -3,9 +3,15 @@
 This is the engine behind the reference's driver loops
 (``cavity_opt1.py:338-347``, ``shear_wave_opt1.py:170-174``): two population
 buffers ping-pong through the fused pull stream-collide kernel, one launch
-per step, on one stream.  The MPI decomposition and halo exchange of the
-reference (``cavity_opt1.py:247-329``) have no counterpart: one MI355X holds
-the whole domain.
+per step, on one stream.  By default the populations are kept in the
+row-interleaved layout ``f[k][i][l]`` (nine channel segments of an x-row
+adjacent in HBM), which streams faster than the reference's ``f[i][k][l]``;
+``populations()`` and ``load()`` convert to and from the reference layout,
+and the arithmetic (hence every bit of the state) is the same in both.
+
+The MPI decomposition and halo exchange of the reference
+(``cavity_opt1.py:247-329``) have no counterpart: one MI355X holds the whole
+domain.
 """
 
 import numpy as np
@@ -38,10 +44,11 @@ class Lattice:
     u0 : lid velocity (cavity only).
     dtype : float64 or float32 storage and arithmetic.
     variant : launch shape of the fused step (-1 = library default).
+    layout : ``"rows"`` (default) or ``"soa"`` (the reference's f[i][k][l]).
     """
 
     def __init__(self, nx, ny, omega, bc="cavity", u0=0.1, dtype=torch.float64, device=None,
-                 variant=-1):
+                 variant=-1, layout="rows"):
         nat.require_gpu()
         if bc not in BCS:
             raise ValueError("bc must be 'cavity' or 'periodic'")
@@ -49,8 +56,17 @@ class Lattice:
         self.omega, self.bc, self.u0 = float(omega), bc, float(u0)
         self.dtype = _TORCH_DTYPES[dtype]
         self.device = torch.device(device) if device is not None else torch.device("cuda")
+        if layout not in ("rows", "soa"):
+            raise ValueError("layout must be 'rows' or 'soa'")
+        self.layout = nat.LAYOUT_ROWS if layout == "rows" else nat.LAYOUT_SOA
+        if variant == "auto":
+            from .autotune import step_variant
+            variant = step_variant(self.nx, self.ny, self.dtype, self.layout, BCS[bc])
         self.variant = int(variant)
-        shape = (9, self.nx, self.ny)
+        if self.layout == nat.LAYOUT_ROWS:
+            shape = (self.nx, 9, self.ny)
+        else:
+            shape = (9, self.nx, self.ny)
         self.f = torch.empty(shape, dtype=self.dtype, device=self.device)
         self._g = torch.empty(shape, dtype=self.dtype, device=self.device)
         self.time = 0
@@ -67,21 +83,34 @@ class Lattice:
             t = torch.as_tensor(x, dtype=self.dtype, device=self.device)
             return t.expand(shp).contiguous()
 
-        kernels.equilibrium(field(rho, 1.0), field(ux, 0.0), field(uy, 0.0), out=self.f)
+        soa = kernels.equilibrium(field(rho, 1.0), field(ux, 0.0), field(uy, 0.0))
+        self._from_soa(soa)
         self.time = 0
         return self
 
+    def _from_soa(self, soa):
+        if self.layout == nat.LAYOUT_SOA:
+            self.f.copy_(soa)
+        else:
+            kernels.relayout(soa, nat.LAYOUT_SOA, nat.LAYOUT_ROWS, out=self.f)
+
     def load(self, f):
-        """Replace the populations by ``f`` (numpy or tensor of shape (9, nx, ny))."""
+        """Replace the populations by ``f`` (reference layout, shape (9, nx, ny))."""
         t = torch.as_tensor(np.asarray(f) if not isinstance(f, torch.Tensor) else f)
         if tuple(t.shape) != (9, self.nx, self.ny):
             raise ValueError("populations must have shape (9, %d, %d)" % (self.nx, self.ny))
-        self.f.copy_(t.to(self.dtype))
+        self._from_soa(t.to(device=self.device, dtype=self.dtype).contiguous())
         return self
 
+    def populations_device(self):
+        """Populations in the reference layout (9, nx, ny) as a device tensor."""
+        if self.layout == nat.LAYOUT_SOA:
+            return self.f.clone()
+        return kernels.relayout(self.f, nat.LAYOUT_ROWS, nat.LAYOUT_SOA)
+
     def populations(self):
-        """Populations as a numpy array (a device-to-host copy)."""
-        return self.f.cpu().numpy()
+        """Populations in the reference layout (9, nx, ny) as a numpy array."""
+        return self.populations_device().cpu().numpy()
 
     # -- stepping ------------------------------------------------------------
     def step(self, n=1, moments=None):
@@ -91,7 +120,7 @@ class Lattice:
         for s in range(n):
             mom = moments if (moments is not None and s == n - 1) else None
             kernels.step(self.f, self._g, self.omega, bc, self.u0, moments=mom,
-                         variant=self.variant)
+                         variant=self.variant, layout=self.layout)
             self.f, self._g = self._g, self.f
         self.time += n
         return self
@@ -99,7 +128,7 @@ class Lattice:
     # -- observables ---------------------------------------------------------
     def moments(self):
         """(rho, ux, uy) of the current populations, as device tensors."""
-        return kernels.moments(self.f)
+        return kernels.moments(self.f, layout=self.layout)
 
     def mass(self):
         """Total mass sum_i,k,l f, accumulated in FP64 on the device."""
@@ -109,3 +138,22 @@ class Lattice:
         """(ux, uy) as numpy arrays."""
         _, ux, uy = self.moments()
         return ux.cpu().numpy(), uy.cpu().numpy()
+
+    def run_until_steady(self, tol=1e-6, every=1000, max_steps=10**7, u_ref=None):
+        """Step until max|u(t + every) - u(t)| / u_ref < tol (FP64 only).
+
+        ``u_ref`` defaults to the lid velocity.  Returns the number of steps
+        taken and the last residual.
+        """
+        u_ref = u_ref or self.u0
+        uxp = torch.zeros((self.nx, self.ny), dtype=torch.float64, device=self.device)
+        uyp = torch.zeros_like(uxp)
+        kernels.velocity_residual(self.f, uxp, uyp, layout=self.layout)
+        taken, res = 0, float("inf")
+        while taken < max_steps:
+            self.step(every)
+            taken += every
+            res = float(kernels.velocity_residual(self.f, uxp, uyp, layout=self.layout).item()) / u_ref
+            if res < tol:
+                break
+        return taken, res

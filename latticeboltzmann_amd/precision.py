@@ -1,0 +1,171 @@
+"""Reduced-precision population storage: FP32, BF16 or FP16 deviations.
+
+``DeviationLattice`` keeps ``d_i = scale * (f_i - w_i)`` in HBM in the storage
+format and steps it with an FP32 collide (``lbm_dev_step``).  Halving or
+quartering the bytes per population raises the lattice-update rate of the
+bandwidth-bound step by the same factor.  The error this costs is measured
+against an FP64 run of the same problem by ``velocity_errors``; FP64 is kept
+for the mass and residual reductions.
+
+Default scales: FP16 ``2**10`` (keeps deviations down to ~6e-8 out of the
+FP16 subnormal range while the largest cavity deviations, ~0.1, stay far
+below the FP16 maximum); BF16 and FP32 ``1`` (their exponent range is FP32's).
+
+Rounding of the 16-bit formats (``rounding``): ``"stochastic"`` (default) or
+``"nearest"``; both keep each node's mass and momentum to a single rounding
+error (see ``lbm_dev_step`` in include/lbm_d2q9.h).  Stochastic rounding is
+seeded by ``seed`` and the step counter, so a run is reproducible.
+"""
+
+import torch
+
+from . import _native as nat
+from . import kernels
+
+FORMATS = {"f32": nat.FMT_F32, "bf16": nat.FMT_BF16, "f16": nat.FMT_F16}
+_RAW = {nat.FMT_F32: torch.float32, nat.FMT_BF16: torch.int16, nat.FMT_F16: torch.int16}
+DEFAULT_SCALE = {"f32": 1.0, "bf16": 1.0, "f16": 1024.0}
+BYTES = {"f32": 4, "bf16": 2, "f16": 2}
+ROUNDING = {"nearest": nat.ROUND_NEAREST, "stochastic": nat.ROUND_STOCHASTIC}
+
+
+class DeviationLattice:
+    """A D2Q9 lattice whose populations are stored as scaled deviations.
+
+    Same stepping interface as :class:`latticeboltzmann_amd.lattice.Lattice`.
+    """
+
+    def __init__(self, nx, ny, omega, bc="cavity", u0=0.1, fmt="f16", scale=None, device=None,
+                 variant=-1, layout="rows", rounding="stochastic", seed=0):
+        nat.require_gpu()
+        if fmt not in FORMATS:
+            raise ValueError("fmt must be one of %s" % sorted(FORMATS))
+        if rounding not in ROUNDING:
+            raise ValueError("rounding must be 'nearest' or 'stochastic'")
+        self.rounding = ROUNDING[rounding]
+        self.seed = int(seed) & 0xFFFFFFFF
+        if bc not in ("cavity", "periodic"):
+            raise ValueError("bc must be 'cavity' or 'periodic'")
+        self.nx, self.ny = int(nx), int(ny)
+        self.omega, self.bc, self.u0 = float(omega), bc, float(u0)
+        self.fmt_name, self.fmt = fmt, FORMATS[fmt]
+        self.scale = float(scale if scale is not None else DEFAULT_SCALE[fmt])
+        self.device = torch.device(device) if device is not None else torch.device("cuda")
+        self.layout = nat.LAYOUT_ROWS if layout == "rows" else nat.LAYOUT_SOA
+        self._bc = nat.BC_CAVITY if bc == "cavity" else nat.BC_PERIODIC
+        shape = (self.nx, 9, self.ny) if self.layout == nat.LAYOUT_ROWS else (9, self.nx, self.ny)
+        self.d = torch.empty(shape, dtype=_RAW[self.fmt], device=self.device)
+        self._e = torch.empty_like(self.d)
+        if variant == "auto":
+            variant = self._tune()
+        self.variant = int(variant)
+        self.time = 0
+        self.init_equilibrium()
+
+    def _tune(self):
+        from .autotune import best_variant
+
+        def make_launch(nx, ny):
+            a = torch.zeros_like(self.d)
+            b = torch.empty_like(self.d)
+
+            def launch(v):
+                self._call_step(a, b, v)
+            return launch
+
+        n_var = nat.load().lbm_dev_variant_count()
+        return best_variant("dev_step", self.nx, self.ny, self.fmt_name, self.layout, self._bc,
+                            make_launch, range(n_var))
+
+    # -- state -----------------------------------------------------------------
+    def init_equilibrium(self, rho=None, ux=None, uy=None):
+        shp = (self.nx, self.ny)
+
+        def field(x, default):
+            if x is None:
+                return torch.full(shp, default, dtype=torch.float64, device=self.device)
+            return torch.as_tensor(x, dtype=torch.float64, device=self.device).expand(shp).contiguous()
+
+        self.load(kernels.equilibrium(field(rho, 1.0), field(ux, 0.0), field(uy, 0.0)))
+        self.time = 0
+        return self
+
+    def load(self, f):
+        """Encode FP64 populations (reference layout (9, nx, ny)) into storage."""
+        f = torch.as_tensor(f, device=self.device, dtype=torch.float64).contiguous()
+        if tuple(f.shape) != (9, self.nx, self.ny):
+            raise ValueError("populations must have shape (9, %d, %d)" % (self.nx, self.ny))
+        nat.call("lbm_dev_encode", nat.ptr(f), nat.ptr(self.d), self.nx, self.ny, self.layout,
+                 self.fmt, self.scale, nat.stream_ptr())
+        return self
+
+    def populations_device(self):
+        """Decoded FP64 populations in the reference layout, on the device."""
+        f = torch.empty((9, self.nx, self.ny), dtype=torch.float64, device=self.device)
+        nat.call("lbm_dev_decode", nat.ptr(self.d), nat.ptr(f), self.nx, self.ny, self.layout,
+                 self.fmt, self.scale, nat.stream_ptr())
+        return f
+
+    def populations(self):
+        return self.populations_device().cpu().numpy()
+
+    # -- stepping ----------------------------------------------------------------
+    def _call_step(self, src, dst, variant, ux=None, uy=None, step_index=0):
+        # a different dither stream every step, reproducible from (seed, time)
+        seed = (self.seed * 0x9E3779B1 + step_index * 0x85EBCA77 + 1) & 0xFFFFFFFF
+        nat.call("lbm_dev_step", nat.ptr(src), nat.ptr(dst), self.fmt, self.nx, self.ny,
+                 self.layout, self.omega, self._bc, self.u0, self.scale, self.rounding, seed,
+                 nat.ptr(ux), nat.ptr(uy), int(variant), nat.stream_ptr())
+
+    def step(self, n=1):
+        for _ in range(n):
+            self._call_step(self.d, self._e, self.variant, step_index=self.time)
+            self.d, self._e = self._e, self.d
+            self.time += 1
+        return self
+
+    # -- observables ---------------------------------------------------------------
+    def moments(self):
+        """(rho, ux, uy) in FP64 from the decoded populations."""
+        return kernels.moments(self.populations_device())
+
+    def velocity(self):
+        _, ux, uy = self.moments()
+        return ux.cpu().numpy(), uy.cpu().numpy()
+
+    def mass(self):
+        ws = kernels._workspace(self.device)
+        nat.call("lbm_dev_mass", nat.ptr(self.d), self.nx, self.ny, self.fmt, self.scale,
+                 nat.ptr(ws.work), nat.ptr(ws.out), nat.stream_ptr())
+        return float(ws.out.item())
+
+    def run_until_steady(self, tol=1e-6, every=1000, max_steps=10**7, u_ref=None):
+        u_ref = u_ref or self.u0
+        ws = kernels._workspace(self.device)
+        uxp = torch.zeros((self.nx, self.ny), dtype=torch.float64, device=self.device)
+        uyp = torch.zeros_like(uxp)
+
+        def residual():
+            nat.call("lbm_dev_velocity_residual", nat.ptr(self.d), self.nx, self.ny, self.layout,
+                     self.fmt, self.scale, nat.ptr(uxp), nat.ptr(uyp), nat.ptr(ws.work),
+                     nat.ptr(ws.out), nat.stream_ptr())
+            return float(ws.out.item())
+
+        residual()
+        taken, res = 0, float("inf")
+        while taken < max_steps:
+            self.step(every)
+            taken += every
+            res = residual() / u_ref
+            if res < tol:
+                break
+        return taken, res
+
+
+def velocity_errors(ux, uy, ux_ref, uy_ref, u_ref):
+    """RMS and max of |u - u_ref| over the grid, divided by ``u_ref`` (FP64)."""
+    dx = ux.double() - ux_ref.double()
+    dy = uy.double() - uy_ref.double()
+    mag = torch.sqrt(dx * dx + dy * dy)
+    return {"l2_over_U": float(torch.sqrt(torch.mean(mag * mag)).item()) / u_ref,
+            "linf_over_U": float(mag.max().item()) / u_ref}

@@ -138,12 +138,14 @@ def test_collide_moments_and_large_n(gpu):
         assert np.array_equal(host(a).reshape(b.shape), b)
 
 
-def test_moments_bitexact(gpu):
-    from latticeboltzmann_amd import kernels
+def test_moments_bitexact_both_layouts(gpu):
+    from latticeboltzmann_amd import _native, kernels
     f = populations(50, 70)
-    rho, ux, uy = kernels.moments(dev(f))
-    for a, b in zip((rho, ux, uy), o.moments(f)):
-        assert np.array_equal(host(a), b)
+    for layout, t in ((_native.LAYOUT_SOA, dev(f)),
+                      (_native.LAYOUT_ROWS, dev(f.transpose(1, 0, 2).copy()))):
+        rho, ux, uy = kernels.moments(t, layout=layout)
+        for a, b in zip((rho, ux, uy), o.moments(f)):
+            assert np.array_equal(host(a), b)
 
 
 # ---------------------------------------------------------------------------
@@ -158,22 +160,27 @@ def variants():
 @pytest.mark.parametrize("shape", SHAPES)
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("bc", ["cavity", "periodic"])
-def test_fused_step_bitexact_all_variants(gpu, shape, dtype, bc):
+def test_fused_step_bitexact_all_variants_and_layouts(gpu, shape, dtype, bc):
     from latticeboltzmann_amd import _native, kernels
     f = populations(*shape, dtype)
     expect, rho_e, ux_e, uy_e = cref.step(f, 1.7, bc, 0.1)
-    src = dev(f)
     kind = _native.BC_CAVITY if bc == "cavity" else _native.BC_PERIODIC
-    for v in variants():
-        dst = torch.empty_like(src)
-        mom = tuple(torch.empty(shape, dtype=src.dtype, device=gpu) for _ in range(3))
-        kernels.step(src, dst, 1.7, kind, 0.1, moments=mom, variant=v)
-        assert np.array_equal(host(dst), expect), "variant %d" % v
-        for a, b in zip(mom, (rho_e, ux_e, uy_e)):
-            assert np.array_equal(host(a), b), "variant %d moments" % v
-        dst2 = torch.empty_like(src)
-        kernels.step(src, dst2, 1.7, kind, 0.1, variant=v)
-        assert torch.equal(dst, dst2)
+    soa = dev(f)
+    rows = kernels.relayout(soa, _native.LAYOUT_SOA, _native.LAYOUT_ROWS)
+    assert np.array_equal(host(rows), f.transpose(1, 0, 2))
+    for layout, src in ((_native.LAYOUT_SOA, soa), (_native.LAYOUT_ROWS, rows)):
+        for v in variants():
+            dst = torch.empty_like(src)
+            mom = tuple(torch.empty(shape, dtype=src.dtype, device=gpu) for _ in range(3))
+            kernels.step(src, dst, 1.7, kind, 0.1, moments=mom, variant=v, layout=layout)
+            out = dst if layout == _native.LAYOUT_SOA else kernels.relayout(
+                dst, _native.LAYOUT_ROWS, _native.LAYOUT_SOA)
+            assert np.array_equal(host(out), expect), "layout %d variant %d" % (layout, v)
+            for a, b in zip(mom, (rho_e, ux_e, uy_e)):
+                assert np.array_equal(host(a), b), "layout %d variant %d moments" % (layout, v)
+            dst2 = torch.empty_like(src)
+            kernels.step(src, dst2, 1.7, kind, 0.1, variant=v, layout=layout)
+            assert torch.equal(dst, dst2)
 
 
 def test_dropin_cavity_step_sequence_bitexact(gpu):
@@ -188,21 +195,22 @@ def test_dropin_cavity_step_sequence_bitexact(gpu):
     assert np.array_equal(f, g)
 
 
-def test_cavity_crop_512_bitexact_200_steps(gpu):
+@pytest.mark.parametrize("layout", ["rows", "soa"])
+def test_cavity_crop_512_bitexact_200_steps(gpu, layout):
     # the full-step comparison of the 2048^2 cavity workload on its 512^2 crop:
     # Re = 1000, U = 0.1 -> omega = 1/(3 * 0.1 * 512/1000 + 1/2)
     from latticeboltzmann_amd.lattice import Lattice, omega_for_cavity
     n, steps = 512, 200
     omega = omega_for_cavity(n, 1000.0, 0.1)
     assert abs(omega - 1.529988) < 1e-6
-    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64)
+    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, layout=layout)
     f0 = lat.populations()
     assert np.array_equal(f0, o.cavity_init(n, n))
     lat.step(steps)
     expect = cref.run(f0, omega, steps, "cavity", 0.1)
     got = lat.populations()
     assert np.array_equal(got, expect)
-    # the contract's tolerance, stated: full steps within 1e-12 relative
+    # the tolerance the comparison is held to: full steps within 1e-12 relative
     assert np.abs(got - expect).max() <= 1e-12 * np.abs(expect).max()
 
 
@@ -243,16 +251,18 @@ def test_sum_fp64_accurate_and_deterministic(gpu):
 
 
 def test_velocity_residual(gpu):
-    from latticeboltzmann_amd import kernels
+    from latticeboltzmann_amd import _native, kernels
     f = populations(40, 30)
-    uxp = dev(RNG.random((40, 30)) * 0.01)
-    uyp = dev(RNG.random((40, 30)) * 0.01)
     _, ux, uy = o.moments(f)
-    expect = np.sqrt((ux - host(uxp)) ** 2 + (uy - host(uyp)) ** 2).max()
-    r = float(kernels.velocity_residual(dev(f), uxp, uyp).item())
-    assert abs(r - expect) <= 1e-15 * expect
-    assert np.array_equal(host(uxp), ux) and np.array_equal(host(uyp), uy)
-    assert float(kernels.velocity_residual(dev(f), uxp, uyp).item()) == 0.0
+    for layout, t in ((_native.LAYOUT_SOA, dev(f)),
+                      (_native.LAYOUT_ROWS, dev(f.transpose(1, 0, 2).copy()))):
+        uxp = dev(RNG.random((40, 30)) * 0.01)
+        uyp = dev(RNG.random((40, 30)) * 0.01)
+        expect = np.sqrt((ux - host(uxp)) ** 2 + (uy - host(uyp)) ** 2).max()
+        r = float(kernels.velocity_residual(t, uxp, uyp, layout=layout).item())
+        assert abs(r - expect) <= 1e-15 * expect
+        assert np.array_equal(host(uxp), ux) and np.array_equal(host(uyp), uy)
+        assert float(kernels.velocity_residual(t, uxp, uyp, layout=layout).item()) == 0.0
 
 
 # ---------------------------------------------------------------------------
@@ -272,7 +282,9 @@ def test_cavity_2048_properties(gpu):
     # with the lid: deterministic (bitwise) across runs and launch variants
     a = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=0).step(40)
     b = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=5).step(40)
+    c = Lattice(n, n, omega, "cavity", 0.1, torch.float64, layout="soa").step(40)
     assert torch.equal(a.f, b.f)
+    assert torch.equal(a.populations_device(), c.f)
     _, ux, _ = a.moments()
     assert float(ux[:, -1].mean()) > 0.0
     assert torch.isfinite(a.f).all()
