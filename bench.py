@@ -48,7 +48,30 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
     ap.add_argument("--cpu-budget", type=float, default=8.0)
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only time lbm_relayout (a plain copy of the 9*n*n FP64 populations) "
+                         "--steps times: the known-size copy that scales the HBM counters")
     return ap.parse_args()
+
+
+def copy_only(n, reps):
+    """Time the relayout kernel on an n x n FP64 population array (read + write 144 B/node)."""
+    import torch
+    from latticeboltzmann_amd import _native, kernels
+    a = torch.zeros((9, n, n), dtype=torch.float64, device="cuda")
+    b = torch.empty((n, 9, n), dtype=torch.float64, device="cuda")
+    kernels.relayout(a, _native.LAYOUT_SOA, _native.LAYOUT_ROWS, out=b)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(reps):
+        kernels.relayout(a, _native.LAYOUT_SOA, _native.LAYOUT_ROWS, out=b)
+    end.record()
+    torch.cuda.synchronize()
+    sec = start.elapsed_time(end) / 1e3 / reps
+    nbytes = 2 * a.numel() * 8
+    print(json.dumps({"copy": "lbm_relayout", "n": n, "bytes_per_launch": nbytes,
+                      "us_per_launch": round(sec * 1e6, 2), "gbs": round(nbytes / sec / 1e9, 1)}))
 
 
 def timed_steps(lat, steps, warmup):
@@ -125,6 +148,9 @@ def main():
     _native.require_gpu()
     _native.load()
     n = args.n
+    if args.copy_only:
+        copy_only(n, args.steps)
+        return
     omega = omega_for_cavity(n, 1000.0, 0.1)
     if args.variant != "auto":
         args.variant = int(args.variant)
