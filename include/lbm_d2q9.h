@@ -166,12 +166,19 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
  * seed.  FP32 storage always rounds to nearest.                               */
 #define LBM_ROUND_NEAREST 0
 #define LBM_ROUND_STOCHASTIC 1
+/* Compute type of the collide: FP32 (the reduced-precision form) or FP64
+ * (FP32 storage only; other formats return LBM_E_ARG).  With FP64 compute,
+ * FP32 storage is rounded like the 16-bit formats (compensated, nearest or
+ * stochastic); with FP32 compute FP32 storage needs no rounding and
+ * `rounding` is ignored for it.                                               */
+#define LBM_COMPUTE_F32 0
+#define LBM_COMPUTE_F64 1
 /* One fused step src -> dst on stored deviations; ux/uy (nullable, FP32
  * [k][l]) receive the pre-collision velocity.  variant in
  * 0 .. lbm_dev_variant_count()-1 or -1.                                         */
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                 float omega, int bc, float u0, float scale, int rounding, uint32_t seed,
-                 float *ux, float *uy, int variant, void *stream);
+                 float omega, int bc, float u0, float scale, int compute, int rounding,
+                 uint32_t seed, float *ux, float *uy, int variant, void *stream);
 /* *out = sum_{i,k,l} f = nx*ny + (sum of stored values) / scale, in FP64     */
 int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,
                  double *out, void *stream);
@@ -179,6 +186,61 @@ int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, d
 int lbm_dev_velocity_residual(const void *d, int64_t nx, int64_t ny, int layout, int fmt,
                               double scale, double *ux_prev, double *uy_prev, double *work,
                               double *out, void *stream);
+
+/* ---- batched instances ------------------------------------------------------------
+ * Many independent lattices of different sizes in one population pool, each
+ * stored as deviations in the row-interleaved layout at its own offset, and
+ * stepped together: one launch per step covers every instance of a launch
+ * group (a persistent grid looping over node tiles).  No reference
+ * counterpart: the reference runs one lattice per process
+ * (shear_wave_opt1.py:43-53, cavity_opt1.py:51-59).                           */
+typedef struct lbm_batch_inst {
+    int64_t offset; /* element offset of the instance's 9*nx*ny values in the pool */
+    int32_t nx, ny;
+    float omega;
+    float u0;    /* lid velocity when bc == LBM_BC_CAVITY */
+    int32_t bc;  /* LBM_BC_PERIODIC or LBM_BC_CAVITY */
+    int32_t dir; /* observable: 0 = amplitude of u_x along y, 1 = of u_y along x */
+} lbm_batch_inst;
+
+typedef struct lbm_batch_tile {
+    int32_t inst;  /* instance index */
+    int32_t first; /* first node k*ny + l of the tile */
+} lbm_batch_tile;
+
+int lbm_batch_variant_count(void);
+/* Nodes per tile of a batched launch shape (threads per block x nodes per thread). */
+int lbm_batch_tile_nodes(int variant);
+/* One step of every tile in tiles[0..ntiles) (tile size lbm_batch_tile_nodes(variant)),
+ * src pool -> dst pool.  blocks = 0 picks the persistent grid size.  The dither
+ * seed of stochastic rounding is seed combined with *epoch (device counter,
+ * nullable), so graph replays get fresh streams.                               */
+int lbm_batch_step(const void *src, void *dst, int fmt, float scale,
+                   const lbm_batch_inst *insts, const lbm_batch_tile *tiles, int ntiles,
+                   int compute, int rounding, uint32_t seed, const uint32_t *epoch, int variant,
+                   int blocks, void *stream);
+/* Shear-wave amplitude of every instance by projection on its initial sine,
+ * A = 2/(nx ny) sum u_dir sin(2 pi pos / n), FP64; written to
+ * amp[*epoch * ninst + i] when *epoch < nslots.  tiles: 256-node tiles in
+ * instance order; inst_tiles[i] .. inst_tiles[i+1] are instance i's tiles;
+ * partial holds ntiles doubles.                                               */
+int lbm_batch_amplitude(const void *pool, int fmt, float scale, const lbm_batch_inst *insts,
+                        int ninst, const lbm_batch_tile *tiles, int ntiles,
+                        const int32_t *inst_tiles, double *partial, double *amp, int nslots,
+                        const uint32_t *epoch, void *stream);
+/* *counter += 1 on the device (a one-thread kernel, safe inside a graph).     */
+int lbm_counter_increment(uint32_t *counter, void *stream);
+/* *counter = value on the device (a one-thread kernel).                      */
+int lbm_counter_set(uint32_t *counter, uint32_t value, void *stream);
+
+/* ---- HIP graphs ---------------------------------------------------------------------
+ * Capture the work enqueued on a (non-default) stream between begin and end
+ * into an executable graph, then replay it with one launch.                   */
+typedef struct lbm_graph lbm_graph;
+int lbm_graph_begin(void *stream);
+int lbm_graph_end(void *stream, lbm_graph **graph);
+int lbm_graph_launch(lbm_graph *graph, void *stream);
+int lbm_graph_destroy(lbm_graph *graph);
 
 #ifdef __cplusplus
 }

@@ -29,6 +29,7 @@ LAYOUT_SOA = 0   # f[i][k][l], the reference's f_ikl
 LAYOUT_ROWS = 1  # f[k][i][l], row-interleaved
 FMT_F32, FMT_BF16, FMT_F16 = 0, 1, 2
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
+COMPUTE_F32, COMPUTE_F64 = 0, 1
 
 _vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
 _f64, _f32 = ctypes.c_double, ctypes.c_float
@@ -62,10 +63,24 @@ SIGNATURES = {
     "lbm_dev_encode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
     "lbm_dev_decode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
     "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32, _i32,
-                            ctypes.c_uint32, _vp, _vp, _i32, _vp]),
+                            _i32, ctypes.c_uint32, _vp, _vp, _i32, _vp]),
     "lbm_dev_mass": (_i32, [_vp, _i64, _i64, _i32, _f64, _vp, _vp, _vp]),
     "lbm_dev_velocity_residual": (_i32, [_vp, _i64, _i64, _i32, _i32, _f64, _vp, _vp, _vp, _vp,
                                          _vp]),
+    # batched instances
+    "lbm_batch_variant_count": (_i32, []),
+    "lbm_batch_tile_nodes": (_i32, [_i32]),
+    "lbm_batch_step": (_i32, [_vp, _vp, _i32, _f32, _vp, _vp, _i32, _i32, _i32, ctypes.c_uint32,
+                              _vp, _i32, _i32, _vp]),
+    "lbm_batch_amplitude": (_i32, [_vp, _i32, _f32, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _i32,
+                                   _vp, _vp]),
+    "lbm_counter_increment": (_i32, [_vp, _vp]),
+    "lbm_counter_set": (_i32, [_vp, ctypes.c_uint32, _vp]),
+    # HIP graphs
+    "lbm_graph_begin": (_i32, [_vp]),
+    "lbm_graph_end": (_i32, [_vp, ctypes.POINTER(ctypes.c_void_p)]),
+    "lbm_graph_launch": (_i32, [_vp, _vp]),
+    "lbm_graph_destroy": (_i32, [_vp]),
 }
 
 _lib = None

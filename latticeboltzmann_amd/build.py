@@ -8,6 +8,10 @@ flags and the objects are linked into one shared library next to this file.
   may the kernels that must reproduce it.
 * every unit is compiled with the ``packed-fp32-ops`` target feature switched
   off, so the compiler never emits packed FP32 arithmetic.
+* the reduced-precision units are compiled with ``-fno-slp-vectorize``: with
+  the SLP vectorizer on, the ROCm 7.2 gfx950 backend aborts on the FP16 step
+  kernels ("Cannot select: v5i32 = bitcast", from packing the nine 16-bit
+  outputs of a node); the kernels are bandwidth-bound, so nothing is lost.
 
 Usage: ``python -m latticeboltzmann_amd.build`` (or ``build()`` from Python).
 """
@@ -32,8 +36,8 @@ COMMON = [
 # per-unit flags
 UNITS = {
     "d2q9_kernels.hip": ["-ffp-contract=off"],
-    "d2q9_lowprec.hip": ["-ffp-contract=fast"],
-    "d2q9_batch.hip": ["-ffp-contract=off"],
+    "d2q9_lowprec.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
+    "d2q9_batch.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
     "surrogate.hip": ["-ffp-contract=fast"],
 }
 

@@ -15,192 +15,19 @@
 // encode/decode.  No reference counterpart: the reference stores f itself
 // in FP32 or FP64 (cavity_opt1.py:59).
 
+#include "d2q9_dev.hpp"
 #include "d2q9_grid.hpp"
 #include "d2q9_reduce.hpp"
 #include "lbm_internal.hpp"
 
 namespace lbm {
 
-__host__ __device__ constexpr float wf(int i) {
-    return i == 0 ? 4.0f / 9.0f : (i <= 4 ? 1.0f / 9.0f : 1.0f / 36.0f);
-}
-__host__ __device__ constexpr double wd(int i) {
-    return i == 0 ? 4.0 / 9.0 : (i <= 4 ? 1.0 / 9.0 : 1.0 / 36.0);
-}
-
-// Storage codecs: raw storage type and conversions.  enc rounds an FP32 value
-// to the format, enc64 rounds an FP64 value to the format directly (one
-// rounding), both to nearest with ties to even.
-template <int FMT>
-struct Codec;
-template <>
-struct Codec<LBM_FMT_F32> {
-    using raw = float;
-    __device__ static float dec(raw x) { return x; }
-    __device__ static raw enc(float x) { return x; }
-    __device__ static raw enc64(double x) { return float(x); }
-};
-template <>
-struct Codec<LBM_FMT_BF16> {
-    using raw = uint16_t;
-    __device__ static float dec(raw x) { return float(__builtin_bit_cast(__bf16, x)); }
-    __device__ static raw enc(float x) { return __builtin_bit_cast(uint16_t, __bf16(x)); }
-    __device__ static raw enc64(double x) { return __builtin_bit_cast(uint16_t, __bf16(x)); }
-};
-template <>
-struct Codec<LBM_FMT_F16> {
-    using raw = uint16_t;
-    __device__ static float dec(raw x) { return float(__builtin_bit_cast(_Float16, x)); }
-    __device__ static raw enc(float x) { return __builtin_bit_cast(uint16_t, _Float16(x)); }
-    __device__ static raw enc64(double x) { return __builtin_bit_cast(uint16_t, _Float16(x)); }
-};
-
-// Stochastic rounding: x is rounded up in magnitude with probability equal
-// to its distance from the value below, so the rounding error has zero mean.
-// Deterministic rounding of a slowly varying flow rounds every node the same
-// way every step, which biases conserved quantities (a shear wave then decays
-// at the wrong rate); stochastic rounding turns that bias into noise.
-// Implemented on the FP32 bit pattern: add 16 (BF16) or 13 (FP16) random
-// bits below the kept significand and truncate.  FP16 values outside the
-// FP16 normal range fall back to nearest rounding.
-__device__ __forceinline__ uint32_t hash32(uint32_t x) {
-    x ^= x >> 16;
-    x *= 0x7feb352dU;
-    x ^= x >> 15;
-    x *= 0x846ca68bU;
-    x ^= x >> 16;
-    return x;
-}
-
-// Per-node random stream: one hash of (seed, node), then an LCG per value
-// whose high 16 bits supply the dither.
-struct Dither {
-    uint32_t s;
-    __device__ __forceinline__ Dither(uint32_t seed, uint64_t node)
-        : s(hash32(seed ^ hash32(uint32_t(node) ^ (uint32_t(node >> 32) * 0x9e3779b9U)))) {}
-    __device__ __forceinline__ uint32_t next16() {
-        s = s * 1664525U + 1013904223U;
-        return s >> 16;
-    }
-};
-
-template <int FMT>
-__device__ __forceinline__ typename Codec<FMT>::raw enc_sr(float x, uint32_t r16);
-template <>
-__device__ __forceinline__ float enc_sr<LBM_FMT_F32>(float x, uint32_t) {
-    return x;
-}
-template <>
-__device__ __forceinline__ uint16_t enc_sr<LBM_FMT_BF16>(float x, uint32_t r16) {
-    const uint32_t b = __builtin_bit_cast(uint32_t, x);
-    if ((b & 0x7f800000U) == 0x7f800000U) return Codec<LBM_FMT_BF16>::enc(x);  // inf / nan
-    return uint16_t((b + r16) >> 16);
-}
-template <>
-__device__ __forceinline__ uint16_t enc_sr<LBM_FMT_F16>(float x, uint32_t r16) {
-    const float a = fabsf(x);
-    if (!(a >= 6.103515625e-05f && a < 65504.0f)) return Codec<LBM_FMT_F16>::enc(x);
-    uint32_t b = __builtin_bit_cast(uint32_t, x);
-    b = (b + (r16 & 0x1fffU)) & ~0x1fffU;  // now exactly representable in FP16
-    return Codec<LBM_FMT_F16>::enc(__builtin_bit_cast(float, b));
-}
-
-// Cavity walls on deviations (same rules and order as the FP64 kernel).
-template <int FMT>
-__device__ __forceinline__ void cavity_walls_dev(float g[9], const typename Codec<FMT>::raw *c,
-                                                 int64_t cs, int k, int l, int nx, int ny,
-                                                 float u0, float inv_scale) {
-    using C = Codec<FMT>;
-    if (l == 0) {
-        g[2] = C::dec(c[4 * cs]) * inv_scale;
-        g[5] = C::dec(c[7 * cs]) * inv_scale;
-        g[6] = C::dec(c[8 * cs]) * inv_scale;
-    }
-    if (l == ny - 1) {
-        const float c2 = C::dec(c[2 * cs]) * inv_scale;
-        const float c5 = C::dec(c[5 * cs]) * inv_scale;
-        const float c6 = C::dec(c[6 * cs]) * inv_scale;
-        const float rl = 1.0f + (((c6 + c2) + (c5 + g[6])) + ((g[2] + g[5]) + (g[3] + (g[0] + g[1]))));
-        const float t = (6.0f / 36.0f) * rl * u0;
-        g[4] = c2;
-        g[8] = c6 + t;
-        g[7] = c5 - t;
-    }
-    if (k == 0) {
-        g[1] = C::dec(c[3 * cs]) * inv_scale;
-        g[5] = C::dec(c[7 * cs]) * inv_scale;
-        g[8] = C::dec(c[6 * cs]) * inv_scale;
-    }
-    if (k == nx - 1) {
-        g[3] = C::dec(c[1 * cs]) * inv_scale;
-        g[6] = C::dec(c[8 * cs]) * inv_scale;
-        g[7] = C::dec(c[5 * cs]) * inv_scale;
-    }
-}
-
-// BGK collide on deviations in FP32.
-__device__ __forceinline__ void collide_dev(float d[9], float omega, float &ux, float &uy) {
-    const float drho = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + (d[7] + d[8])));
-    const float rho = 1.0f + drho;
-    const float jx = ((d[1] - d[3]) + (d[5] - d[6])) + (d[8] - d[7]);
-    const float jy = ((d[2] - d[4]) + (d[5] + d[6])) - (d[7] + d[8]);
-    const float inv = 1.0f / rho;
-    ux = jx * inv;
-    uy = jy * inv;
-    const float uu15 = 1.5f * (ux * ux + uy * uy);
-    float cu[9];
-    cu[0] = 0.0f;
-    cu[1] = ux;
-    cu[2] = uy;
-    cu[3] = -ux;
-    cu[4] = -uy;
-    cu[5] = ux + uy;
-    cu[6] = uy - ux;
-    cu[7] = -(ux + uy);
-    cu[8] = ux - uy;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const float deq = wf(i) * (drho + rho * ((3.0f * cu[i] + 4.5f * (cu[i] * cu[i])) - uu15));
-        d[i] = d[i] + omega * (deq - d[i]);
-    }
-}
-
-// Round the nine scaled post-collision deviations x[] to the storage format so
-// that the stored mass and momentum stay as close as possible to the exact
-// ones.  Rounding each value independently leaves the conserved moments with
-// the sum of nine rounding errors, which collisions never relax, so they
-// random-walk from step to step.  Instead the diagonals (5-8) and E, N are
-// rounded first; W and S are rounded at targets shifted by the x- and
-// y-momentum residuals so far; the rest population, which does not stream,
-// absorbs the mass residual.  Each conserved moment then carries the
-// rounding error of a single value.  Residuals are exact in FP32 (x and the
-// rounded value differ by less than one storage ulp).
-template <int FMT, bool SR>
-__device__ __forceinline__ void round_conserving(const float x[9], typename Codec<FMT>::raw r[9],
-                                                 Dither &dither) {
-    using C = Codec<FMT>;
-    auto rnd = [&](int i, float target) {
-        if constexpr (SR) r[i] = enc_sr<FMT>(target, dither.next16());
-        else r[i] = C::enc(target);
-        return target - C::dec(r[i]);
-    };
-    const float e5 = rnd(5, x[5]), e6 = rnd(6, x[6]), e7 = rnd(7, x[7]), e8 = rnd(8, x[8]);
-    const float e1 = rnd(1, x[1]), e2 = rnd(2, x[2]);
-    const float ex = ((e5 - e6) + (e8 - e7)) + e1;  // x-momentum stored short by ex
-    const float ey = ((e5 + e6) - (e7 + e8)) + e2;  // y-momentum stored short by ey
-    const float e3 = rnd(3, x[3] - ex);
-    const float e4 = rnd(4, x[4] - ey);
-    // mass residual of channels 1..8: e_i for plain roundings, e3 + ex and e4 + ey
-    // for the shifted ones
-    const float em = ((e5 + e6) + (e7 + e8)) + ((e1 + e2) + ((e3 + ex) + (e4 + ey)));
-    rnd(0, x[0] + em);
-}
-
-template <int FMT, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
+// CT is the compute type of the collide: float (the default form) or double.
+template <int FMT, typename CT, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
 __global__ __launch_bounds__(BS) void k_dev_step(const typename Codec<FMT>::raw *__restrict__ src,
                                                  typename Codec<FMT>::raw *__restrict__ dst, int nx,
-                                                 int ny, Layout L, int nseg, float omega, float u0,
-                                                 float scale, float inv_scale, int stochastic,
+                                                 int ny, Layout L, int nseg, CT omega, CT u0,
+                                                 CT scale, CT inv_scale, int stochastic,
                                                  uint32_t seed, float *__restrict__ ux_o,
                                                  float *__restrict__ uy_o) {
     using C = Codec<FMT>;
@@ -222,32 +49,23 @@ __global__ __launch_bounds__(BS) void k_dev_step(const typename Codec<FMT>::raw 
     for (int v = 0; v < V; ++v) {
         const int l = l0 + v * BS;
         if (l >= ny) continue;
-        float g[9];
+        CT g[9];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = C::dec(raw[v][i]) * inv_scale;
+        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[v][i])) * inv_scale;
         if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls_dev<FMT>(g, src + rk + l, L.cs, k, l, nx, ny, u0, inv_scale);
-        float ux, uy;
-        collide_dev(g, omega, ux, uy);
+            cavity_walls_dev<FMT, CT>(g, src + rk + l, L.cs, k, l, nx, ny, u0, inv_scale);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
         R *d = dst + rk + l;
         R out[9];
-        if constexpr (FMT == LBM_FMT_F32) {
-#pragma unroll
-            for (int i = 0; i < 9; ++i) out[i] = C::enc(g[i] * scale);
-        } else {
-            float x[9];
-#pragma unroll
-            for (int i = 0; i < 9; ++i) x[i] = g[i] * scale;
-            Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
-            if (stochastic) round_conserving<FMT, true>(x, out, dither);
-            else round_conserving<FMT, false>(x, out, dither);
-        }
+        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+        store_values<FMT, CT>(g, scale, stochastic, dither, out);
 #pragma unroll
         for (int i = 0; i < 9; ++i) st<NTS>(d + i * L.cs, out[i]);
         if constexpr (MOM) {
             const int64_t j = int64_t(k) * ny + l;
-            ux_o[j] = ux;
-            uy_o[j] = uy;
+            ux_o[j] = float(ux);
+            uy_o[j] = float(uy);
         }
     }
 }
@@ -350,27 +168,28 @@ inline int64_t fmt_bytes(int fmt) { return fmt == LBM_FMT_F32 ? 4 : 2; }
 struct DevArgs {
     int nx, ny;
     Layout L;
-    float omega, u0, scale;
+    double omega, u0, scale;
     int stochastic;
     uint32_t seed;
     float *ux, *uy;
 };
 
-template <int FMT, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
+template <int FMT, typename CT, int BC, bool MOM, int BS, int V, bool NTL, bool NTS>
 void launch_dev_v(const void *src, void *dst, const DevArgs &a, hipStream_t s) {
     using R = typename Codec<FMT>::raw;
     const int nseg = (a.ny + V * BS - 1) / (V * BS);
-    hipLaunchKernelGGL((k_dev_step<FMT, BC, MOM, BS, V, NTL, NTS>),
+    hipLaunchKernelGGL((k_dev_step<FMT, CT, BC, MOM, BS, V, NTL, NTS>),
                        dim3(unsigned(int64_t(a.nx) * nseg)), dim3(BS), 0, s,
                        static_cast<const R *>(src), static_cast<R *>(dst), a.nx, a.ny, a.L, nseg,
-                       a.omega, a.u0, a.scale, 1.0f / a.scale, a.stochastic, a.seed, a.ux, a.uy);
+                       CT(a.omega), CT(a.u0), CT(a.scale), CT(1) / CT(a.scale), a.stochastic,
+                       a.seed, a.ux, a.uy);
 }
 
-template <int FMT, int BC, bool MOM>
+template <int FMT, typename CT, int BC, bool MOM>
 void launch_dev_bc(const void *src, void *dst, const DevArgs &a, int shape, hipStream_t s) {
-#define LBM_DEV_LAUNCH(id, bs, v, ntl, nts)                          \
-    case id:                                                        \
-        launch_dev_v<FMT, BC, MOM, bs, v, ntl, nts>(src, dst, a, s); \
+#define LBM_DEV_LAUNCH(id, bs, v, ntl, nts)                              \
+    case id:                                                            \
+        launch_dev_v<FMT, CT, BC, MOM, bs, v, ntl, nts>(src, dst, a, s); \
         break;
     switch (shape) {
         LBM_DEV_SHAPES(LBM_DEV_LAUNCH)
@@ -379,16 +198,30 @@ void launch_dev_bc(const void *src, void *dst, const DevArgs &a, int shape, hipS
 #undef LBM_DEV_LAUNCH
 }
 
-template <int FMT>
+template <int FMT, typename CT>
 void launch_dev_fmt(const void *src, void *dst, int bc, const DevArgs &a, int shape,
                     hipStream_t s) {
     const bool mom = a.ux != nullptr;
     if (bc == LBM_BC_CAVITY) {
-        if (mom) launch_dev_bc<FMT, LBM_BC_CAVITY, true>(src, dst, a, shape, s);
-        else launch_dev_bc<FMT, LBM_BC_CAVITY, false>(src, dst, a, shape, s);
+        if (mom) launch_dev_bc<FMT, CT, LBM_BC_CAVITY, true>(src, dst, a, shape, s);
+        else launch_dev_bc<FMT, CT, LBM_BC_CAVITY, false>(src, dst, a, shape, s);
     } else {
-        if (mom) launch_dev_bc<FMT, LBM_BC_PERIODIC, true>(src, dst, a, shape, s);
-        else launch_dev_bc<FMT, LBM_BC_PERIODIC, false>(src, dst, a, shape, s);
+        if (mom) launch_dev_bc<FMT, CT, LBM_BC_PERIODIC, true>(src, dst, a, shape, s);
+        else launch_dev_bc<FMT, CT, LBM_BC_PERIODIC, false>(src, dst, a, shape, s);
+    }
+}
+
+// FP32 compute serves every format; FP64 compute is built for FP32 storage only.
+void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc,
+                        const DevArgs &a, int shape, hipStream_t s) {
+    if (compute == LBM_COMPUTE_F64) {
+        launch_dev_fmt<LBM_FMT_F32, double>(src, dst, bc, a, shape, s);
+        return;
+    }
+    switch (fmt) {
+        case LBM_FMT_F32: launch_dev_fmt<LBM_FMT_F32, float>(src, dst, bc, a, shape, s); break;
+        case LBM_FMT_BF16: launch_dev_fmt<LBM_FMT_BF16, float>(src, dst, bc, a, shape, s); break;
+        default: launch_dev_fmt<LBM_FMT_F16, float>(src, dst, bc, a, shape, s); break;
     }
 }
 
@@ -459,13 +292,15 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
 }
 
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                 float omega, int bc, float u0, float scale, int rounding, uint32_t seed,
-                 float *ux, float *uy, int variant, void *stream) {
+                 float omega, int bc, float u0, float scale, int compute, int rounding,
+                 uint32_t seed, float *ux, float *uy, int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if ((ux == nullptr) != (uy == nullptr)) return LBM_E_NULL;
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
     if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0f)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
+    if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
+    if (compute == LBM_COMPUTE_F64 && fmt != LBM_FMT_F32) return LBM_E_ARG;
     if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
     if (variant < -1 || variant >= kNumDevShapes) return LBM_E_ARG;
     const int64_t bytes = 9 * nx * ny * fmt_bytes(fmt);
@@ -475,11 +310,7 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
     const DevArgs a{int(nx), int(ny), make_layout(layout, nx, ny), omega, u0, scale,
                     rounding == LBM_ROUND_STOCHASTIC ? 1 : 0, seed, ux, uy};
     hipStream_t s = as_stream(stream);
-    switch (fmt) {
-        case LBM_FMT_F32: launch_dev_fmt<LBM_FMT_F32>(src, dst, bc, a, shape, s); break;
-        case LBM_FMT_BF16: launch_dev_fmt<LBM_FMT_BF16>(src, dst, bc, a, shape, s); break;
-        default: launch_dev_fmt<LBM_FMT_F16>(src, dst, bc, a, shape, s); break;
-    }
+    launch_dev_compute(src, dst, fmt, compute, bc, a, shape, s);
     return launch_status();
 }
 

@@ -1,0 +1,103 @@
+"""Batched instances: bitwise equal to standalone runs, graph replay equal to
+eager launches, amplitude and fitted viscosity against the analytic decay."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import d2q9_numpy as o
+
+pytestmark = pytest.mark.gpu
+
+MIXED = [
+    dict(nx=64, ny=1024, omega=1.2, amplitude=0.05, direction="x", bc="periodic"),
+    dict(nx=1024, ny=64, omega=0.7, amplitude=0.02, direction="y", bc="periodic"),
+    dict(nx=128, ny=256, omega=1.9, amplitude=0.1, direction="y", bc="periodic"),
+    dict(nx=64, ny=64, omega=0.3, amplitude=0.01, direction="x", bc="periodic"),
+    dict(nx=192, ny=128, omega=1.5, amplitude=0.0, direction="x", bc="cavity", u0=0.1),
+    dict(nx=512, ny=512, omega=1.0, amplitude=0.08, direction="x", bc="periodic"),
+]
+
+
+@pytest.mark.parametrize("compute", ["f32", "f64"])
+def test_batch_bitwise_equals_standalone(gpu, compute):
+    # each instance, stepped inside the batch, has the bits of the same lattice
+    # stepped alone from the same stored state (deterministic rounding)
+    from latticeboltzmann_amd.batch import BatchedLattices
+    from latticeboltzmann_amd.precision import DeviationLattice
+    steps = 30
+    for variant in ("auto", 3, 6):
+        bat = BatchedLattices(MIXED, fmt="f32", variant=variant, compute=compute,
+                              rounding="nearest")
+        alone = []
+        for i, p in enumerate(MIXED):
+            off, n = int(bat.table["offset"][i]), 9 * p["nx"] * p["ny"]
+            lat = DeviationLattice(p["nx"], p["ny"], p["omega"], p["bc"], p.get("u0", 0.0),
+                                   fmt="f32", rounding="nearest", compute=compute)
+            lat.d.copy_(bat.a[off:off + n].view(p["nx"], 9, p["ny"]))
+            alone.append(lat.step(steps))
+        bat.step(steps)
+        for i, (p, lat) in enumerate(zip(MIXED, alone)):
+            off, n = int(bat.table["offset"][i]), 9 * p["nx"] * p["ny"]
+            got = bat.a[off:off + n].view(p["nx"], 9, p["ny"])
+            assert torch.equal(got, lat.d), (variant, i)
+        # and the batch's initial state is the oracle's shear wave to FP32 rounding
+        f0 = BatchedLattices(MIXED[:1], fmt="f32", variant=0).populations(0)
+        p = MIXED[0]
+        ref = o.shear_wave_init(p["nx"], p["ny"], p["amplitude"], p["direction"])
+        assert np.abs(f0 - ref).max() < 1e-8
+
+
+def test_graph_replay_equals_eager(gpu):
+    from latticeboltzmann_amd.batch import BatchedLattices
+    a = BatchedLattices(MIXED, fmt="f32", variant=1, seed=5)
+    b = BatchedLattices(MIXED, fmt="f32", variant=1, seed=5)
+    ta, amp_a = a.run(120, sample_every=40, graph=True)
+    tb, amp_b = b.run(120, sample_every=40, graph=False)
+    assert np.array_equal(ta, tb) and ta[-1] == 120
+    assert torch.equal(a.a, b.a)
+    assert np.array_equal(amp_a, amp_b)
+    # graph replays again from the new state, with fresh sample slots
+    ta2, amp_a2 = a.run(80, sample_every=40, graph=True)
+    b.run(80, sample_every=40, graph=False)
+    assert torch.equal(a.a, b.a) and ta2[0] == 120
+
+
+def test_amplitude_is_the_projection(gpu):
+    from latticeboltzmann_amd.batch import BatchedLattices
+    bat = BatchedLattices(MIXED, fmt="f32", variant=0)
+    times, amp = bat.run(40, sample_every=20, graph=False)
+    for i, p in enumerate(MIXED):
+        if p["bc"] != "periodic":
+            continue
+        assert abs(amp[0, i] - p["amplitude"]) < 1e-6 * p["amplitude"]
+        f = bat.populations(i)
+        # the oracle projects one line; a pure shear wave gives the same value
+        line = o.shear_wave_amplitude(f, p["direction"])
+        assert abs(amp[-1, i] - line) < 1e-5 * p["amplitude"], i
+
+
+def test_fp16_stochastic_batch_viscosity(gpu):
+    from latticeboltzmann_amd.batch import BatchedLattices
+    inst = [dict(nx=128, ny=128, omega=w, amplitude=0.05, direction=d, bc="periodic")
+            for w in (0.6, 1.0, 1.4) for d in ("x", "y")]
+    bat = BatchedLattices(inst, fmt="f16", rounding="stochastic", seed=3)
+    times, amp = bat.run(1000, sample_every=100)
+    nu = bat.viscosities(times, amp)
+    assert np.all(np.abs(nu / bat.theory() - 1) < 0.02), nu / bat.theory()
+
+
+def test_slow_decay_viscosity_unbiased_with_fp64_compute(gpu):
+    # a long-wavelength, low-viscosity shear wave changes by ~20 FP32 ulps per
+    # step: an FP32 collide biases nu by ~10 %, FP64 compute + stochastic
+    # rounding to FP32 storage does not
+    from latticeboltzmann_amd.batch import BatchedLattices
+    inst = [dict(nx=1024, ny=64, omega=1.82, amplitude=0.086, direction="y", bc="periodic"),
+            dict(nx=64, ny=1024, omega=1.78, amplitude=0.05, direction="x", bc="periodic")]
+    errs = {}
+    for compute, rounding in (("f32", "nearest"), ("f64", "stochastic")):
+        bat = BatchedLattices(inst, fmt="f32", compute=compute, rounding=rounding)
+        times, amp = bat.run(2000, sample_every=50)
+        errs[compute] = np.abs(bat.viscosities(times, amp) / bat.theory() - 1)
+    assert errs["f64"].max() < 5e-3, errs
+    assert errs["f32"].max() > 3 * errs["f64"].max(), errs

@@ -1,0 +1,54 @@
+"""Host-side logic that needs no GPU: sweep draws, tile tables, table layouts."""
+
+import ctypes
+
+import numpy as np
+
+
+def test_sweep_draws_are_reproducible():
+    from latticeboltzmann_amd.batch import SIZES, shear_wave_sweep
+    a, b = shear_wave_sweep(512, 0), shear_wave_sweep(512, 0)
+    assert a == b and len(a) == 512
+    assert {p["nx"] for p in a} <= set(SIZES) and {p["ny"] for p in a} <= set(SIZES)
+    assert all(0.3 <= p["omega"] <= 1.9 and 0.01 <= p["amplitude"] <= 0.1 for p in a)
+    assert {p["direction"] for p in a} == {"x", "y"}
+    # the draws of numpy.random.default_rng(0), in the documented order
+    rng = np.random.default_rng(0)
+    nx = rng.choice(SIZES, 512)
+    assert [p["nx"] for p in a] == nx.tolist()
+    assert shear_wave_sweep(512, 1) != a
+
+
+def test_instance_and_tile_records_match_the_c_structs():
+    from latticeboltzmann_amd.batch import INST_DTYPE, TILE_DTYPE
+
+    class Inst(ctypes.Structure):
+        _fields_ = [("offset", ctypes.c_int64), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
+                    ("omega", ctypes.c_float), ("u0", ctypes.c_float), ("bc", ctypes.c_int32),
+                    ("dir", ctypes.c_int32)]
+
+    class Tile(ctypes.Structure):
+        _fields_ = [("inst", ctypes.c_int32), ("first", ctypes.c_int32)]
+
+    assert INST_DTYPE.itemsize == ctypes.sizeof(Inst) == 32
+    assert TILE_DTYPE.itemsize == ctypes.sizeof(Tile) == 8
+    for name, _ in Inst._fields_:
+        assert INST_DTYPE.fields[name][1] == getattr(Inst, name).offset
+
+
+def test_tiles_cover_every_node_once():
+    from latticeboltzmann_amd.batch import _tiles
+    sizes = [64 * 64, 64 * 1024, 128 * 192]
+    for tile in (64, 256, 1024):
+        t = _tiles(range(3), sizes, tile)
+        for i, n in enumerate(sizes):
+            firsts = np.sort(t["first"][t["inst"] == i])
+            assert firsts[0] == 0 and np.all(np.diff(firsts) == tile)
+            assert firsts[-1] < n <= firsts[-1] + tile
+
+
+def test_size_class_rounds_up_to_powers_of_two():
+    from latticeboltzmann_amd.autotune import size_class
+    assert size_class(1000, 1000) == "1024x1024"
+    assert size_class(64, 65) == "64x128"
+    assert size_class(1, 1) == "1x1"
