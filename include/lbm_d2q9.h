@@ -137,7 +137,8 @@ int lbm_sum_f64(const double *x, int64_t n, double *work, double *out, void *str
 int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stream);
 /* Velocity residual: out = max_j |(ux, uy)(f)_j - (ux_prev, uy_prev)_j| over
  * the nx*ny nodes of an FP64 population array f in the given layout;
- * ux_prev/uy_prev ([k][l]) are then overwritten with the current velocity.   */
+ * ux_prev/uy_prev ([k][l]) are then overwritten with the current velocity.
+ * A node whose difference is NaN (a diverged lattice) makes out = +infinity. */
 int lbm_velocity_residual_f64(const double *f, int64_t nx, int64_t ny, int layout,
                               double *ux_prev, double *uy_prev, double *work,
                               double *out, void *stream);
@@ -182,7 +183,8 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
 /* *out = sum_{i,k,l} f = nx*ny + (sum of stored values) / scale, in FP64     */
 int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,
                  double *out, void *stream);
-/* velocity residual of stored deviations (velocity evaluated in FP64)        */
+/* velocity residual of stored deviations (velocity evaluated in FP64); NaN
+ * anywhere gives +infinity, as lbm_velocity_residual_f64                     */
 int lbm_dev_velocity_residual(const void *d, int64_t nx, int64_t ny, int layout, int fmt,
                               double scale, double *ux_prev, double *uy_prev, double *work,
                               double *out, void *stream);
@@ -241,6 +243,64 @@ int lbm_graph_begin(void *stream);
 int lbm_graph_end(void *stream, lbm_graph **graph);
 int lbm_graph_launch(lbm_graph *graph, void *stream);
 int lbm_graph_destroy(lbm_graph *graph);
+
+/* ---- Warm-start surrogate --------------------------------------------------------------
+ * No counterpart in the reference (it has no learned component).  Kernels of a
+ * small convolutional encoder-decoder (3x3 same convolutions as im2col + GEMM on
+ * BF16 MFMA with FP32 accumulation, 2x2 average pooling, nearest 2x
+ * upsampling, channel concatenation) with their backward passes, an MSE loss
+ * and Adam, plus the initializer that turns the network's prediction into
+ * populations.  Activations are NHWC; BF16 buffers are raw 16-bit words.
+ * Every reduction runs in a fixed order, so results are deterministic.        */
+
+/* Bytes of the split-K workspace lbm_nn_gemm needs (0 when ksplit <= 1).     */
+int64_t lbm_nn_gemm_workspace(int M, int N, int ksplit);
+/* C[m*ldc + n] (+)= sum_k A[m*sam + k*sak] B[k*sbk + n*sbn]  (A, B BF16),
+ * then + bias[n] (nullable), then max(., 0) if relu; written to C (FP32,
+ * nullable) and/or Cb (BF16, nullable).  accumulate adds the old C.  ksplit > 1
+ * splits K over that many block planes into work (lbm_nn_gemm_workspace bytes)
+ * and sums them in order.                                                    */
+int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
+                float *C, void *Cb, int64_t ldc, const float *bias, int M, int N, int K,
+                int accumulate, int relu, int ksplit, float *work, void *stream);
+/* cols[((b*H + y)*W + x)*9*C + (dy*3 + dx)*C + c] = x[b][y+dy-1][x+dx-1][c] (0 outside). */
+int lbm_nn_im2col3x3(const void *x, int B, int H, int W, int C, void *cols, void *stream);
+/* Adjoint of im2col3x3 (a gather, FP32): dx (+)= col2im(dcols).              */
+int lbm_nn_col2im3x3(const float *dcols, int B, int H, int W, int C, float *dx, int accumulate,
+                     void *stream);
+/* dy *= (y > 0), with an optional BF16 copy of the result.                   */
+int lbm_nn_relu_backward(const float *y, float *dy, void *dy_bf16, int64_t n, void *stream);
+/* db[n] = sum_m dy[m*N + n] (FP64 accumulation).                             */
+int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, void *stream);
+/* 2x2 average pool x [B][H][W][C] -> y [B][H/2][W/2][C] (+ optional BF16 copy). */
+int lbm_nn_avgpool2(const float *x, int B, int H, int W, int C, float *y, void *yb, void *stream);
+int lbm_nn_avgpool2_backward(const float *dy, int B, int H, int W, int C, float *dx,
+                             int accumulate, void *stream);
+/* Nearest 2x upsample x [B][H][W][C] into channels [coff, coff+C) of
+ * y [B][2H][2W][Ct] (+ optional BF16 copy); backward sums the four children. */
+int lbm_nn_upsample2(const float *x, int B, int H, int W, int C, float *y, void *yb, int Ct,
+                     int coff, void *stream);
+int lbm_nn_upsample2_backward(const float *dy, int B, int H, int W, int C, int Ct, int coff,
+                              float *dx, void *stream);
+/* dst[p*Cd + doff + c] (+)= src[p*Cs + soff + c], c < C, p < P (+ BF16 copy). */
+int lbm_nn_copy_channels(const float *src, int Cs, int soff, float *dst, void *dst_bf16, int Cd,
+                         int doff, int C, int64_t P, int accumulate, void *stream);
+/* loss = mean((y - t)^2) (FP64, via work of lbm_reduce_workspace_bytes);
+ * dy = 2 (y - t) / n.                                                        */
+int lbm_nn_mse(const float *y, const float *t, int64_t n, float *dy, double *work, double *loss,
+               void *stream);
+/* One Adam update (bias-corrected for step >= 1) of FP32 p, with BF16 copy.  */
+int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int64_t n, float lr,
+                float b1, float b2, float eps, int step, void *stream);
+int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream);
+/* Warm-start populations (FP64, layout f[i][k][l]) from a prediction
+ * q[m][m][3] = ((rho-1)/U^2, ux/U, uy/U) on the cell centres of an m x m grid
+ * over the cavity: bilinear interpolation to the nodes (clamped at the
+ * outermost centres), f = f_eq(rho, u); with neq, plus the first-order
+ * non-equilibrium part of a post-collision BGK state, (1 - omega) * (-3 w_i
+ * rho / omega) (c_i c_i - 1/3) : S, S the strain rate of the interpolated u. */
+int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, double omega,
+                      int neq, double *f, void *stream);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,24 @@ SIGNATURES = {
     "lbm_graph_end": (_i32, [_vp, ctypes.POINTER(ctypes.c_void_p)]),
     "lbm_graph_launch": (_i32, [_vp, _vp]),
     "lbm_graph_destroy": (_i32, [_vp]),
+    # warm-start surrogate
+    "lbm_nn_gemm_workspace": (_i64, [_i32, _i32, _i32]),
+    "lbm_nn_gemm": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _i32, _i32,
+                           _i32, _i32, _i32, _i32, _vp, _vp]),
+    "lbm_nn_im2col3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp]),
+    "lbm_nn_col2im3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
+    "lbm_nn_relu_backward": (_i32, [_vp, _vp, _vp, _i64, _vp]),
+    "lbm_nn_bias_grad": (_i32, [_vp, _i64, _i32, _vp, _vp]),
+    "lbm_nn_avgpool2": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
+    "lbm_nn_avgpool2_backward": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
+    "lbm_nn_upsample2": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp]),
+    "lbm_nn_upsample2_backward": (_i32, [_vp, _i32, _i32, _i32, _i32, _i32, _i32, _vp, _vp]),
+    "lbm_nn_copy_channels": (_i32, [_vp, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _i64, _i32,
+                                    _vp]),
+    "lbm_nn_mse": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "lbm_nn_adam": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _vp]),
+    "lbm_nn_to_bf16": (_i32, [_vp, _vp, _i64, _vp]),
+    "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _vp, _vp]),
 }
 
 _lib = None

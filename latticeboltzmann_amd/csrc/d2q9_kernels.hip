@@ -256,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restr
         double rho, ux, uy;
         moments_opt1(g, rho, ux, uy);
         const double dx = ux - uxp[j], dy = uy - uyp[j];
-        m = fmax(m, sqrt(dx * dx + dy * dy));
+        m = monitor_max(m, sqrt(dx * dx + dy * dy));
         uxp[j] = ux;
         uyp[j] = uy;
     }

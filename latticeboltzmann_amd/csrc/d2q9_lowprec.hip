@@ -133,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_dev_residual_pass1(
         const double ux = (((g[1] - g[3]) + (g[5] - g[6])) + (g[8] - g[7])) / rho;
         const double uy = (((g[2] - g[4]) + (g[5] + g[6])) - (g[7] + g[8])) / rho;
         const double dx = ux - uxp[j], dy = uy - uyp[j];
-        m = fmax(m, sqrt(dx * dx + dy * dy));
+        m = monitor_max(m, sqrt(dx * dx + dy * dy));
         uxp[j] = ux;
         uyp[j] = uy;
     }

@@ -31,6 +31,12 @@ __device__ __forceinline__ double block_sum(double v) {
     return red[0];
 }
 
+// max(m, v) for a convergence monitor: a NaN v (a diverged node) becomes
+// +infinity, which every later fmax keeps, so the maximum cannot lose it.
+__device__ __forceinline__ double monitor_max(double m, double v) {
+    return fmax(m, v == v ? v : __builtin_inf());
+}
+
 __device__ __forceinline__ double block_max(double v) {
     __shared__ double red[kBlock];
     red[threadIdx.x] = v;

@@ -1,0 +1,553 @@
+// Kernels of the warm-start surrogate: a small convolutional encoder-decoder
+// trained and evaluated on the GPU, plus the initializer that turns its
+// prediction into populations.
+//
+// Convolutions are 3x3, stride 1, zero padding, NHWC activations.  They run
+// as im2col + GEMM: cols[pixel][(dy*3+dx)*C + c] (BF16) times the weights
+// W[cout][(dy*3+dx)*C + c] (BF16), accumulated in FP32 by v_mfma_f32_16x16x32_bf16
+// (gfx950).  Backward: dW = dY^T cols (split-K over pixels with a fixed-order
+// reduction), dcols = dY W, dX = col2im(dcols) as a gather (no atomics), so
+// every result is deterministic.  Activations are kept in FP32 with a BF16
+// copy for the next GEMM; weights in FP32 (Adam) with a BF16 copy.
+//
+// No reference counterpart: the reference has no learned component.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d2q9_node.hpp"
+#include "d2q9_reduce.hpp"
+#include "lbm_internal.hpp"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+namespace lbm {
+namespace nn {
+
+__device__ __forceinline__ float bf2f(uint16_t x) { return float(__builtin_bit_cast(__bf16, x)); }
+__device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(uint16_t, __bf16(x)); }
+
+// ---------------------------------------------------------------------------
+// GEMM: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak],
+// B(k, n) = B[k*sbk + n*sbn] (BF16), FP32 accumulation.
+// Block: 256 threads = 4 waves, tile 64 x 64, k-step 32 staged in LDS; each
+// wave owns a 32 x 32 quarter = 2 x 2 MFMA 16x16x32 tiles.
+// MFMA operand layout (checked on gfx950): lane l holds A rows l%16,
+// k = (l/16)*8 + i; B cols l%16, k = (l/16)*8 + i; result register r of lane l
+// is row (l/16)*4 + r, col l%16.
+// Split-K: blockIdx.z handles k in [z*kchunk, (z+1)*kchunk); with more than
+// one split, results go to work[z][m][n] and k_gemm_reduce adds them in order.
+constexpr int TM = 64, TN = 64, TK = 32;
+
+struct GemmArgs {
+    const uint16_t *A;
+    int64_t sam, sak;
+    const uint16_t *B;
+    int64_t sbk, sbn;
+    float *C;      // FP32 output (nullable)
+    uint16_t *Cb;  // BF16 output (nullable)
+    int64_t ldc;
+    const float *bias;
+    int M, N, K, kchunk, accumulate, relu;
+    float *work;   // split-K partials
+};
+
+__global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
+    __shared__ uint16_t As[TM][TK + 8];
+    __shared__ uint16_t Bs[TK][TN + 8];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = blockIdx.z * g.kchunk;
+    const int kend = min(g.K, kbeg + g.kchunk);
+    const int wm = (w >> 1) * 32, wn = (w & 1) * 32;
+    floatx4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        {  // A tile 64 x 32: thread t -> row t/4, k (t%4)*8 .. +8
+            const int r = t >> 2, kk = (t & 3) * 8;
+            const int m = m0 + r;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int k = k0 + kk + i;
+                As[r][kk + i] = (m < g.M && k < kend) ? g.A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0;
+            }
+        }
+        {  // B tile 32 x 64: thread t -> k t/8, n (t%8)*8 .. +8
+            const int kr = t >> 3, nn = (t & 7) * 8;
+            const int k = k0 + kr;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int n = n0 + nn + i;
+                Bs[kr][nn + i] = (k < kend && n < g.N) ? g.B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0;
+            }
+        }
+        __syncthreads();
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                a[i][e] = __builtin_bit_cast(__bf16, As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8 + e]);
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                b[j][e] = __builtin_bit_cast(__bf16, Bs[(lane >> 4) * 8 + e][wn + j * 16 + (lane & 15)]);
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+
+    const bool split = gridDim.z > 1;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+                const int n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][r];
+                if (split) {
+                    g.work[(int64_t(blockIdx.z) * g.M + m) * g.N + n] = v;
+                    continue;
+                }
+                const int64_t o = int64_t(m) * g.ldc + n;
+                if (g.accumulate) v += g.C[o];
+                if (g.bias) v += g.bias[n];
+                if (g.relu) v = fmaxf(v, 0.0f);
+                if (g.C) g.C[o] = v;
+                if (g.Cb) g.Cb[o] = f2bf(v);
+            }
+}
+
+// Fixed-order sum of the split-K partials plus the epilogue.
+__global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs g, int splits) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= int64_t(g.M) * g.N) return;
+    const int m = int(idx / g.N), n = int(idx - int64_t(m) * g.N);
+    float v = 0.0f;
+    for (int z = 0; z < splits; ++z) v += g.work[int64_t(z) * g.M * g.N + idx];
+    const int64_t o = int64_t(m) * g.ldc + n;
+    if (g.accumulate) v += g.C[o];
+    if (g.bias) v += g.bias[n];
+    if (g.relu) v = fmaxf(v, 0.0f);
+    if (g.C) g.C[o] = v;
+    if (g.Cb) g.Cb[o] = f2bf(v);
+}
+
+// im2col for 3x3 same convolution: x [B][H][W][C] BF16 -> cols [B*H*W][9*C].
+__global__ __launch_bounds__(256) void k_im2col(const uint16_t *__restrict__ x, int B, int H, int W,
+                                                int C, uint16_t *__restrict__ cols) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t total = int64_t(B) * H * W * 9 * C;
+    if (idx >= total) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;
+    const int tap = int(r % 9);
+    r /= 9;  // pixel index
+    const int xw = int(r % W);
+    const int64_t r2 = r / W;
+    const int yh = int(r2 % H);
+    const int b = int(r2 / H);
+    const int yy = yh + tap / 3 - 1, xx = xw + tap % 3 - 1;
+    uint16_t v = 0;
+    if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = x[((int64_t(b) * H + yy) * W + xx) * C + c];
+    cols[idx] = v;
+}
+
+// col2im (gather): dx[b][y][x][c] (+)= sum over taps of dcols[pixel(y - dy + 1, x - dx + 1)][tap*C + c].
+__global__ __launch_bounds__(256) void k_col2im(const float *__restrict__ dcols, int B, int H, int W,
+                                                int C, float *__restrict__ dx, int accumulate) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int64_t total = int64_t(B) * H * W * C;
+    if (idx >= total) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;
+    const int xw = int(r % W);
+    const int64_t r2 = r / W;
+    const int yh = int(r2 % H);
+    const int b = int(r2 / H);
+    float s = 0.0f;
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) {
+        const int py = yh - (tap / 3 - 1), px = xw - (tap % 3 - 1);  // output pixel that read (yh, xw) at this tap
+        if (py < 0 || py >= H || px < 0 || px >= W) continue;
+        s += dcols[(((int64_t(b) * H + py) * W + px) * 9 + tap) * C + c];
+    }
+    dx[idx] = accumulate ? dx[idx] + s : s;
+}
+
+// dy *= (y > 0); writes a BF16 copy of the masked gradient.
+__global__ __launch_bounds__(256) void k_relu_backward(const float *__restrict__ y, float *__restrict__ dy,
+                                                       uint16_t *__restrict__ dyb, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float g = y[i] > 0.0f ? dy[i] : 0.0f;
+    dy[i] = g;
+    if (dyb) dyb[i] = f2bf(g);
+}
+
+// db[n] = sum_m dy[m][n], one block per column, fixed order.
+__global__ __launch_bounds__(kBlock) void k_bias_grad(const float *__restrict__ dy, int64_t M, int N,
+                                                      float *__restrict__ db) {
+    const int n = blockIdx.x;
+    double s = 0.0;
+    for (int64_t m = threadIdx.x; m < M; m += kBlock) s += double(dy[m * N + n]);
+    const double tot = block_sum(s);
+    if (threadIdx.x == 0) db[n] = float(tot);
+}
+
+// 2x2 average pool (NHWC): y [B][H/2][W/2][C].
+__global__ __launch_bounds__(256) void k_avgpool2(const float *__restrict__ x, int B, int H, int W, int C,
+                                                  float *__restrict__ y, uint16_t *__restrict__ yb) {
+    const int Ho = H / 2, Wo = W / 2;
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= int64_t(B) * Ho * Wo * C) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;
+    const int xo = int(r % Wo);
+    const int64_t r2 = r / Wo;
+    const int yo = int(r2 % Ho);
+    const int b = int(r2 / Ho);
+    const float *p = x + ((int64_t(b) * H + 2 * yo) * W + 2 * xo) * C + c;
+    const float v = 0.25f * ((p[0] + p[C]) + (p[int64_t(W) * C] + p[int64_t(W) * C + C]));
+    y[idx] = v;
+    if (yb) yb[idx] = f2bf(v);
+}
+
+__global__ __launch_bounds__(256) void k_avgpool2_backward(const float *__restrict__ dy, int B, int H,
+                                                           int W, int C, float *__restrict__ dx,
+                                                           int accumulate) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= int64_t(B) * H * W * C) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;
+    const int xw = int(r % W);
+    const int64_t r2 = r / W;
+    const int yh = int(r2 % H);
+    const int b = int(r2 / H);
+    const float g = 0.25f * dy[((int64_t(b) * (H / 2) + yh / 2) * (W / 2) + xw / 2) * C + c];
+    dx[idx] = accumulate ? dx[idx] + g : g;
+}
+
+// Nearest 2x upsample of x [B][H][W][C] into channels [coff, coff + C) of
+// y [B][2H][2W][Ct].
+__global__ __launch_bounds__(256) void k_upsample2(const float *__restrict__ x, int B, int H, int W,
+                                                   int C, float *__restrict__ y,
+                                                   uint16_t *__restrict__ yb, int Ct, int coff) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    const int Ho = 2 * H, Wo = 2 * W;
+    if (idx >= int64_t(B) * Ho * Wo * C) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;  // output pixel
+    const int xo = int(r % Wo);
+    const int64_t r2 = r / Wo;
+    const int yo = int(r2 % Ho);
+    const int b = int(r2 / Ho);
+    const float v = x[((int64_t(b) * H + yo / 2) * W + xo / 2) * C + c];
+    const int64_t o = r * Ct + coff + c;
+    y[o] = v;
+    if (yb) yb[o] = f2bf(v);
+}
+
+__global__ __launch_bounds__(256) void k_upsample2_backward(const float *__restrict__ dy, int B, int H,
+                                                            int W, int C, int Ct, int coff,
+                                                            float *__restrict__ dx) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= int64_t(B) * H * W * C) return;
+    const int c = int(idx % C);
+    int64_t r = idx / C;
+    const int xw = int(r % W);
+    const int64_t r2 = r / W;
+    const int yh = int(r2 % H);
+    const int b = int(r2 / H);
+    const int Wo = 2 * W;
+    float s = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int yo = 2 * yh + q / 2, xo = 2 * xw + q % 2;
+        s += dy[((int64_t(b) * (2 * H) + yo) * Wo + xo) * Ct + coff + c];
+    }
+    dx[idx] = s;
+}
+
+// Channel-slice copy: dst[p][doff + c] (+)= src[p][soff + c] for c < C.
+__global__ __launch_bounds__(256) void k_copy_channels(const float *__restrict__ src, int Cs, int soff,
+                                                       float *__restrict__ dst, uint16_t *__restrict__ dstb,
+                                                       int Cd, int doff, int C, int64_t P, int accumulate) {
+    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (idx >= P * C) return;
+    const int c = int(idx % C);
+    const int64_t p = idx / C;
+    float v = src[p * Cs + soff + c];
+    const int64_t o = p * Cd + doff + c;
+    if (accumulate) v += dst[o];
+    dst[o] = v;
+    if (dstb) dstb[o] = f2bf(v);
+}
+
+// Mean squared error over n values: loss partials and dy = 2 (y - t) / n.
+__global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ y, const float *__restrict__ t,
+                                                int64_t n, float *__restrict__ dy,
+                                                double *__restrict__ partial) {
+    double s = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += stride) {
+        const float d = y[i] - t[i];
+        s += double(d) * double(d);
+        dy[i] = 2.0f * d / float(n);
+    }
+    const double tot = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot / double(n);
+}
+
+// Adam on FP32 master weights, BF16 copy for the GEMMs.
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                              float *__restrict__ m, float *__restrict__ v,
+                                              uint16_t *__restrict__ pb, int64_t n, float lr, float b1,
+                                              float b2, float eps, float c1, float c2) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (pb) pb[i] = f2bf(pi);
+}
+
+__global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y,
+                                                 int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Warm start: the surrogate's normalized fields q [m][m][3] =
+// ((rho - 1) / U^2, ux / U, uy / U) on cell centres of an m x m grid covering
+// the cavity -> populations f (FP64, reference layout 9 x nx x ny) on nx x ny
+// nodes.  Bilinear interpolation at node centres ((k + 1/2) m / nx - 1/2,
+// clamped to the outermost cell centres), then f = f_eq(rho, u) in the opt1
+// order; with neq, plus the first-order non-equilibrium part of the BGK
+// post-collision state, (1 - omega) * (-3 w_i rho / omega) Q_i : S, with the
+// strain rate S from central differences of the interpolated velocity
+// (one-sided at the walls).
+
+__device__ __forceinline__ double bilerp(const float *__restrict__ q, int m, int ch, double px, double py) {
+    px = fmin(fmax(px, 0.0), double(m - 1));
+    py = fmin(fmax(py, 0.0), double(m - 1));
+    int i0 = min(int(px), m - 2 < 0 ? 0 : m - 2), j0 = min(int(py), m - 2 < 0 ? 0 : m - 2);
+    if (m == 1) i0 = j0 = 0;
+    const int i1 = min(i0 + 1, m - 1), j1 = min(j0 + 1, m - 1);
+    const double tx = px - i0, ty = py - j0;
+    const double a = q[(int64_t(i0) * m + j0) * 3 + ch], b = q[(int64_t(i1) * m + j0) * 3 + ch];
+    const double c = q[(int64_t(i0) * m + j1) * 3 + ch], d = q[(int64_t(i1) * m + j1) * 3 + ch];
+    return (1 - tx) * ((1 - ty) * a + ty * c) + tx * ((1 - ty) * b + ty * d);
+}
+
+__device__ __forceinline__ void sample(const float *q, int m, int nx, int ny, int k, int l, double U,
+                                       double &rho, double &ux, double &uy) {
+    const double px = (k + 0.5) * m / nx - 0.5, py = (l + 0.5) * m / ny - 0.5;
+    rho = 1.0 + U * U * bilerp(q, m, 0, px, py);
+    ux = U * bilerp(q, m, 1, px, py);
+    uy = U * bilerp(q, m, 2, px, py);
+}
+
+__global__ __launch_bounds__(kBlock) void k_warm_init(const float *__restrict__ q, int m, int nx, int ny,
+                                                      double U, double omega, int neq,
+                                                      double *__restrict__ f) {
+    const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const int64_t P = int64_t(nx) * ny;
+    if (j >= P) return;
+    const int k = int(j / ny), l = int(j - int64_t(k) * ny);
+    double rho, ux, uy;
+    sample(q, m, nx, ny, k, l, U, rho, ux, uy);
+    double feq[9];
+    equilibrium_opt1(rho, ux, uy, feq);
+    if (neq) {
+        const int km = max(k - 1, 0), kp = min(k + 1, nx - 1), lm = max(l - 1, 0), lp = min(l + 1, ny - 1);
+        double r_, uxm, uym, uxp, uyp, uxl, uyl, uxu, uyu;
+        sample(q, m, nx, ny, km, l, U, r_, uxm, uym);
+        sample(q, m, nx, ny, kp, l, U, r_, uxp, uyp);
+        sample(q, m, nx, ny, k, lm, U, r_, uxl, uyl);
+        sample(q, m, nx, ny, k, lp, U, r_, uxu, uyu);
+        const double hx = double(max(kp - km, 1)), hy = double(max(lp - lm, 1));
+        const double dxux = (uxp - uxm) / hx, dxuy = (uyp - uym) / hx;
+        const double dyux = (uxu - uxl) / hy, dyuy = (uyu - uyl) / hy;
+        const double sxx = dxux, syy = dyuy, sxy = 0.5 * (dyux + dxuy);
+        const double fac = (1.0 - omega) * (-3.0 * rho / omega);
+        const double w[9] = {4.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 9, 1.0 / 36, 1.0 / 36, 1.0 / 36, 1.0 / 36};
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            const double cx_ = cx(i), cy_ = cy(i);
+            const double qs = (cx_ * cx_ - 1.0 / 3) * sxx + 2.0 * cx_ * cy_ * sxy + (cy_ * cy_ - 1.0 / 3) * syy;
+            feq[i] += fac * w[i] * qs;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f[i * P + j] = feq[i];
+}
+
+inline unsigned blocks_for(int64_t n, int per = 256) { return unsigned((n + per - 1) / per); }
+
+}  // namespace nn
+}  // namespace lbm
+
+using namespace lbm;
+using namespace lbm::nn;
+
+extern "C" {
+
+int64_t lbm_nn_gemm_workspace(int M, int N, int ksplit) {
+    return ksplit > 1 ? int64_t(ksplit) * M * N * int64_t(sizeof(float)) : 0;
+}
+
+int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
+                float *C, void *Cb, int64_t ldc, const float *bias, int M, int N, int K,
+                int accumulate, int relu, int ksplit, float *work, void *stream) {
+    if (!A || !B || (!C && !Cb)) return LBM_E_NULL;
+    if (M <= 0 || N <= 0 || K <= 0 || ksplit <= 0) return LBM_E_SIZE;
+    if (accumulate && !C) return LBM_E_ARG;
+    if (ksplit > 1 && !work) return LBM_E_WORKSPACE;
+    int kchunk = (K + ksplit - 1) / ksplit;
+    kchunk = (kchunk + TK - 1) / TK * TK;
+    const int splits = (K + kchunk - 1) / kchunk;
+    GemmArgs g{static_cast<const uint16_t *>(A), sam, sak, static_cast<const uint16_t *>(B), sbk, sbn,
+               C, static_cast<uint16_t *>(Cb), ldc, bias, M, N, K, kchunk, accumulate, relu, work};
+    hipStream_t s = as_stream(stream);
+    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+    if (splits > 1)
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(blocks_for(int64_t(M) * N)), dim3(256), 0, s, g, splits);
+    return launch_status();
+}
+
+int lbm_nn_im2col3x3(const void *x, int B, int H, int W, int C, void *cols, void *stream) {
+    if (!x || !cols) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || C <= 0) return LBM_E_SIZE;
+    const int64_t n = int64_t(B) * H * W * 9 * C;
+    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream),
+                       static_cast<const uint16_t *>(x), B, H, W, C, static_cast<uint16_t *>(cols));
+    return launch_status();
+}
+
+int lbm_nn_col2im3x3(const float *dcols, int B, int H, int W, int C, float *dx, int accumulate,
+                     void *stream) {
+    if (!dcols || !dx) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || C <= 0) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_col2im, dim3(blocks_for(int64_t(B) * H * W * C)), dim3(256), 0,
+                       as_stream(stream), dcols, B, H, W, C, dx, accumulate);
+    return launch_status();
+}
+
+int lbm_nn_relu_backward(const float *y, float *dy, void *dy_bf16, int64_t n, void *stream) {
+    if (!y || !dy) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_relu_backward, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), y, dy,
+                       static_cast<uint16_t *>(dy_bf16), n);
+    return launch_status();
+}
+
+int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, void *stream) {
+    if (!dy || !db) return LBM_E_NULL;
+    if (M <= 0 || N <= 0) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_bias_grad, dim3(N), dim3(kBlock), 0, as_stream(stream), dy, M, N, db);
+    return launch_status();
+}
+
+int lbm_nn_avgpool2(const float *x, int B, int H, int W, int C, float *y, void *yb, void *stream) {
+    if (!x || !y) return LBM_E_NULL;
+    if (B <= 0 || H < 2 || W < 2 || C <= 0 || (H & 1) || (W & 1)) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_avgpool2, dim3(blocks_for(int64_t(B) * (H / 2) * (W / 2) * C)), dim3(256), 0,
+                       as_stream(stream), x, B, H, W, C, y, static_cast<uint16_t *>(yb));
+    return launch_status();
+}
+
+int lbm_nn_avgpool2_backward(const float *dy, int B, int H, int W, int C, float *dx, int accumulate,
+                             void *stream) {
+    if (!dy || !dx) return LBM_E_NULL;
+    if (B <= 0 || H < 2 || W < 2 || C <= 0 || (H & 1) || (W & 1)) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_avgpool2_backward, dim3(blocks_for(int64_t(B) * H * W * C)), dim3(256), 0,
+                       as_stream(stream), dy, B, H, W, C, dx, accumulate);
+    return launch_status();
+}
+
+int lbm_nn_upsample2(const float *x, int B, int H, int W, int C, float *y, void *yb, int Ct, int coff,
+                     void *stream) {
+    if (!x || !y) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || coff < 0 || coff + C > Ct) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_upsample2, dim3(blocks_for(int64_t(B) * 4 * H * W * C)), dim3(256), 0,
+                       as_stream(stream), x, B, H, W, C, y, static_cast<uint16_t *>(yb), Ct, coff);
+    return launch_status();
+}
+
+int lbm_nn_upsample2_backward(const float *dy, int B, int H, int W, int C, int Ct, int coff,
+                              float *dx, void *stream) {
+    if (!dy || !dx) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || coff < 0 || coff + C > Ct) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_upsample2_backward, dim3(blocks_for(int64_t(B) * H * W * C)), dim3(256), 0,
+                       as_stream(stream), dy, B, H, W, C, Ct, coff, dx);
+    return launch_status();
+}
+
+int lbm_nn_copy_channels(const float *src, int Cs, int soff, float *dst, void *dst_bf16, int Cd,
+                         int doff, int C, int64_t P, int accumulate, void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if (C <= 0 || P <= 0 || soff < 0 || doff < 0 || soff + C > Cs || doff + C > Cd) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_copy_channels, dim3(blocks_for(P * C)), dim3(256), 0, as_stream(stream), src,
+                       Cs, soff, dst, static_cast<uint16_t *>(dst_bf16), Cd, doff, C, P, accumulate);
+    return launch_status();
+}
+
+int lbm_nn_mse(const float *y, const float *t, int64_t n, float *dy, double *work, double *loss,
+               void *stream) {
+    if (!y || !t || !dy || !work || !loss) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mse, dim3(kReduceBlocks), dim3(kBlock), 0, s, y, t, n, dy, work);
+    launch_sum_pass2(work, kReduceBlocks, 0.0, loss, s);
+    return launch_status();
+}
+
+int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int64_t n, float lr,
+                float b1, float b2, float eps, int step, void *stream) {
+    if (!p || !g || !m || !v) return LBM_E_NULL;
+    if (n <= 0 || step <= 0) return LBM_E_SIZE;
+    const float c1 = 1.0f - powf(b1, float(step)), c2 = 1.0f - powf(b2, float(step));
+    hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), p, g, m, v,
+                       static_cast<uint16_t *>(p_bf16), n, lr, b1, b2, eps, c1, c2);
+    return launch_status();
+}
+
+int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream) {
+    if (!x || !y) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_to_bf16, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), x,
+                       static_cast<uint16_t *>(y), n);
+    return launch_status();
+}
+
+int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, double omega,
+                      int neq, double *f, void *stream) {
+    if (!q || !f) return LBM_E_NULL;
+    if (m <= 0 || !valid_grid(nx, ny)) return LBM_E_SIZE;
+    if (!(U > 0.0) || !(omega > 0.0 && omega < 2.0)) return LBM_E_ARG;
+    const int64_t P = nx * ny;
+    hipLaunchKernelGGL(k_warm_init, dim3(blocks_for(P, kBlock)), dim3(kBlock), 0, as_stream(stream), q,
+                       m, int(nx), int(ny), U, omega, neq, f);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -24,6 +24,8 @@ then decays 10-15 % too slowly); FP64 compute with stochastic rounding
 removes that bias at the same storage cost.
 """
 
+import math
+
 import torch
 
 from . import _native as nat
@@ -170,6 +172,9 @@ class DeviationLattice:
             res = residual() / u_ref
             if res < tol:
                 break
+            if math.isinf(res) or math.isnan(res):
+                raise FloatingPointError("lattice diverged: non-finite velocity residual "
+                                         "after %d steps" % taken)
         return taken, res
 
 

@@ -1,0 +1,117 @@
+"""Warm-started lid-driven cavity on the GPU.
+
+``Predictor`` wraps a trained ``Surrogate`` (weights in an ``.npz``);
+``warm_populations`` turns a prediction into FP64 populations with the HIP
+initializer; ``CavityStepper`` is the stepper the ``Controller`` drives: a
+FP64 ``Lattice`` whose ``every`` steps between residual checks are replayed
+from one HIP graph.
+"""
+
+import os
+import time
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from .. import kernels
+from ..lattice import Lattice, StepGraph, omega_for_cavity
+from .features import GRID, RE_RANGE, U_RANGE, features
+from .network import Surrogate
+
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_weights.npz")
+
+
+class Predictor:
+    """A trained surrogate plus the normalization it was trained with."""
+
+    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE):
+        self.net, self.grid = net, int(grid)
+        self.re_range, self.u_range = tuple(re_range), tuple(u_range)
+
+    @classmethod
+    def load(cls, path=DEFAULT_WEIGHTS, device=None):
+        with np.load(path, allow_pickle=False) as z:
+            state = {k: z[k] for k in z.files}
+        net = Surrogate.from_state(state, device=device)
+        return cls(net, int(state["grid"]), tuple(state["re_range"]), tuple(state["u_range"]))
+
+    def save(self, path):
+        state = self.net.state_dict()
+        state.update(grid=self.grid, re_range=np.array(self.re_range), u_range=np.array(self.u_range))
+        np.savez(path, **state)
+
+    def inputs(self, params):
+        """(B, m, m, 6) device tensor for a list of (Re, U)."""
+        x = np.stack([features(re, u, self.grid, self.re_range, self.u_range) for re, u in params])
+        return torch.as_tensor(x).to(self.net.device)
+
+    def predict(self, re, u):
+        """(m, m, 3) FP32 device tensor: ((rho - 1)/U^2, ux/U, uy/U) on cell centres."""
+        return self.net.forward(self.inputs([(re, u)]))[0].contiguous()
+
+
+def warm_populations(q, nx, ny, U, omega, neq=True):
+    """FP64 populations (9, nx, ny) on q's device from a prediction q (m, m, 3)."""
+    q = q.contiguous().to(torch.float32)
+    m = q.shape[0]
+    if q.shape != (m, m, 3):
+        raise ValueError("prediction must have shape (m, m, 3)")
+    f = torch.empty((9, nx, ny), dtype=torch.float64, device=q.device)
+    nat.call("lbm_warm_init_f64", q.data_ptr(), m, nx, ny, float(U), float(omega), int(bool(neq)),
+             f.data_ptr(), nat.stream_ptr())
+    return f
+
+
+class CavityStepper:
+    """FP64 N x N cavity at (Re, U) for the ``Controller``; use as a context manager."""
+
+    def __init__(self, n, re, u, predictor=None, neq=True, every=1000, graph=True,
+                 variant="auto", device=None):
+        self.n, self.re, self.u = int(n), float(re), float(u)
+        self.omega = omega_for_cavity(self.n, self.re, self.u)
+        self.predictor, self.neq, self.every = predictor, bool(neq), int(every)
+        self.lat = Lattice(self.n, self.n, self.omega, "cavity", self.u, torch.float64,
+                           device=device, variant=variant)
+        self._graph = StepGraph(self.lat, self.every, enabled=graph)
+        self._advance = None
+        d = self.lat.device
+        self.uxp = torch.zeros((self.n, self.n), dtype=torch.float64, device=d)
+        self.uyp = torch.zeros_like(self.uxp)
+
+    def __enter__(self):
+        self._advance = self._graph.__enter__()
+        return self
+
+    def __exit__(self, *exc):
+        self._graph.__exit__(*exc)
+        self._advance = None
+        return False
+
+    def init_cold(self):
+        self.lat.init_equilibrium()
+
+    def init_warm(self):
+        if self.predictor is None:
+            raise RuntimeError("a warm start needs a predictor")
+        q = self.predictor.predict(self.re, self.u)
+        self.lat.load(warm_populations(q, self.n, self.n, self.u, self.omega, self.neq))
+        self.lat.time = 0
+
+    def advance(self, n):
+        if n == self.every and self._advance is not None:
+            self._advance()
+        else:
+            self.lat.step(n)
+
+    def residual(self):
+        r = kernels.velocity_residual(self.lat.f, self.uxp, self.uyp, layout=self.lat.layout)
+        return float(r.item()) / self.u
+
+    def sync_time(self, t0):
+        torch.cuda.synchronize(self.lat.device)
+        return time.perf_counter() - t0
+
+    def fields(self):
+        """(rho, ux, uy) device tensors of the current state."""
+        return self.lat.moments()

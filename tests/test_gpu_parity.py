@@ -288,3 +288,17 @@ def test_cavity_2048_properties(gpu):
     _, ux, _ = a.moments()
     assert float(ux[:, -1].mean()) > 0.0
     assert torch.isfinite(a.f).all()
+
+
+def test_graph_replayed_steady_run_equals_eager(gpu):
+    # run_until_steady replays a captured 1000-step graph; same bits and the
+    # same stopping step as launching every step
+    from latticeboltzmann_amd.lattice import Lattice, omega_for_cavity
+    n = 64
+    om = omega_for_cavity(n, 100.0, 0.1)
+    a = Lattice(n, n, om, "cavity", 0.1, torch.float64)
+    b = Lattice(n, n, om, "cavity", 0.1, torch.float64)
+    sa, ra = a.run_until_steady(tol=1e-4, every=1000, graph=True)
+    sb, rb = b.run_until_steady(tol=1e-4, every=1000, graph=False)
+    assert sa == sb and ra == rb and a.time == b.time == sa
+    assert torch.equal(a.f, b.f)
