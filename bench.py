@@ -9,7 +9,12 @@ events on the stream the kernels are launched on.
 
 Also reported (``extra``): the same kernel on an 8192 x 8192 cavity (past the
 256 MB last-level cache: the size where an HBM-roofline fraction means
-something) and a device-to-device copy of the same byte count for reference.
+something) in FP64 and with FP32/BF16/FP16 deviation storage; a
+device-to-device copy of the same byte count; the batched sweep of 512
+shear-wave instances (seed 0, FP32 storage, 2000 steps, one persistent
+graph-captured launch per step) with its fitted-viscosity errors; and the
+warm-start time to steady state against the cold start on held-out 256 x 256
+cavities (when trained surrogate weights are present).
 
 ``roofline``: algorithmic bytes per lattice update = 2 * 9 * 8 (read and
 write all nine FP64 populations once) = 144 B; achieved = 144 * nodes /
@@ -47,6 +52,9 @@ def parse():
                     "(an integer, or 'auto' for the autotuner)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep leg")
+    ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--warm-count", type=int, default=3)
     ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--copy-only", action="store_true",
                     help="only time lbm_relayout (a plain copy of the 9*n*n FP64 populations) "
@@ -133,6 +141,53 @@ def cpu_baseline(n, budget):
     }
 
 
+def batch_sweep(steps=2000):
+    """512 shear-wave instances (seed 0) in FP32 storage: throughput and nu errors."""
+    import numpy as np
+    import torch
+    from latticeboltzmann_amd.batch import BatchedLattices, shear_wave_sweep
+    bat = BatchedLattices(shear_wave_sweep(512, 0), fmt="f32")
+    bat.run(20, sample_every=10, graph=True)  # tuning, capture and first-touch outside the timing
+    bat = BatchedLattices(shear_wave_sweep(512, 0), fmt="f32")
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    times, amp = bat.run(steps, sample_every=50, graph=True)
+    b.record()
+    torch.cuda.synchronize()
+    sec = a.elapsed_time(b) / 1e3
+    nodes = int(sum(bat.sizes))
+    rel = np.abs(bat.viscosities(times, amp) / bat.theory() - 1)
+    out = {"instances": 512, "nodes": nodes, "steps": steps, "seconds": round(sec, 4),
+           "mlups": round(nodes * steps / sec / 1e6, 1), "storage": "f32", "compute": bat.compute_name,
+           "nu_rel_err_median": float(np.nanmedian(rel)), "nu_rel_err_p90": float(np.nanpercentile(rel, 90)),
+           "nu_rel_err_max": float(np.nanmax(rel))}
+    del bat
+    torch.cuda.empty_cache()
+    return out
+
+
+def warm_start(count, n=256, seed=2000):
+    """Held-out cavities: cold vs warm-started steps and wall time to steady state."""
+    from latticeboltzmann_amd.warmstart.controller import Controller
+    from latticeboltzmann_amd.warmstart.data import draw_cavities
+    from latticeboltzmann_amd.warmstart.evaluate import evaluate_one, summarize
+    from latticeboltzmann_amd.warmstart.solve import DEFAULT_WEIGHTS, Predictor
+    if not os.path.exists(DEFAULT_WEIGHTS):
+        return None
+    pred = Predictor.load(DEFAULT_WEIGHTS)
+    ctl = Controller(tol=1e-6, every=1000)
+    recs = [evaluate_one(pred, n, re, u, ctl) for re, u in
+            draw_cavities(count, seed, pred.re_range, pred.u_range)]
+    out = summarize(recs)["warm"]
+    out["cavities"] = [{"n": r["n"], "re": round(r["re"], 1), "u": round(r["u"], 4),
+                        "cold_steps": r["cold"]["steps"], "warm_steps": r["warm"]["steps"],
+                        "cold_s": round(r["cold"]["seconds"], 3), "warm_s": round(r["warm"]["seconds"], 3),
+                        "diff_max_over_U": r["warm"]["diff_max"]} for r in recs]
+    out["criterion"] = "max|u(t+1000)-u(t)|/U < 1e-6, FP64, N=%d" % n
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -187,6 +242,10 @@ def main():
         extra["cavity_8192"] = legs
         nbytes = 9 * big * big * 8
         extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
+    if not args.no_batch:
+        extra["batch_shear_512"] = batch_sweep()
+    if not args.no_warm:
+        extra["warm_start"] = warm_start(args.warm_count)
 
     line = {
         "metric": "MLUPS + % HBM roofline, FP64 vs FP16-store; time-to-steady-state cavity",

@@ -294,13 +294,22 @@ int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int6
                 float b1, float b2, float eps, int step, void *stream);
 int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream);
 /* Warm-start populations (FP64, layout f[i][k][l]) from a prediction
- * q[m][m][3] = ((rho-1)/U^2, ux/U, uy/U) on the cell centres of an m x m grid
- * over the cavity: bilinear interpolation to the nodes (clamped at the
- * outermost centres), f = f_eq(rho, u); with neq, plus the first-order
- * non-equilibrium part of a post-collision BGK state, (1 - omega) * (-3 w_i
- * rho / omega) (c_i c_i - 1/3) : S, S the strain rate of the interpolated u. */
+ * q[m][m][3] = ((rho-1)/U^2, ux/U, uy/U), averages over the cells of an m x m
+ * grid over the cavity, reconstructed at the nodes, then f = f_eq(rho, u).
+ * Reconstruction (flags): default bilinear between cell centres, clamped at
+ * the outermost centres; LBM_WARM_WALLS bilinear towards ghost cells mirrored
+ * across the walls (u = 0 on the walls, (U, 0) on the lid, zero density
+ * gradient); LBM_WARM_CUBIC (m >= 3, same ghosts) cell averages turned into
+ * centre values, q - c lap(q) with c = (pool^2 - 1) / (24 pool^2) for averages
+ * over pool x pool nodes (pool = 1: none), then separable Catmull-Rom cubics.
+ * LBM_WARM_NEQ adds the first-order non-equilibrium part of a post-collision
+ * BGK state, (1 - omega) * (-3 w_i rho / omega) (c_i c_i - 1/3) : S, S the
+ * strain rate of the reconstructed u.                                       */
+#define LBM_WARM_NEQ 1
+#define LBM_WARM_WALLS 2
+#define LBM_WARM_CUBIC 4
 int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, double omega,
-                      int neq, double *f, void *stream);
+                      int flags, int pool, double *f, void *stream);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ LAYOUT_ROWS = 1  # f[k][i][l], row-interleaved
 FMT_F32, FMT_BF16, FMT_F16 = 0, 1, 2
 ROUND_NEAREST, ROUND_STOCHASTIC = 0, 1
 COMPUTE_F32, COMPUTE_F64 = 0, 1
+WARM_NEQ, WARM_WALLS, WARM_CUBIC = 1, 2, 4
 
 _vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
 _f64, _f32 = ctypes.c_double, ctypes.c_float
@@ -98,7 +99,7 @@ SIGNATURES = {
     "lbm_nn_mse": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
     "lbm_nn_adam": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _vp]),
     "lbm_nn_to_bf16": (_i32, [_vp, _vp, _i64, _vp]),
-    "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _vp, _vp]),
+    "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _i32, _vp, _vp]),
 }
 
 _lib = None

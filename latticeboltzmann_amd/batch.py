@@ -74,7 +74,7 @@ class BatchedLattices:
         compute = compute or ("f64" if fmt == "f32" else "f32")
         if compute not in COMPUTE or (compute == "f64" and fmt != "f32"):
             raise ValueError("compute must be 'f32', or 'f64' with FP32 storage")
-        self.compute = COMPUTE[compute]
+        self.compute, self.compute_name = COMPUTE[compute], compute
         self.instances = [dict(x) for x in instances]
         self.n = len(self.instances)
         if self.n == 0:

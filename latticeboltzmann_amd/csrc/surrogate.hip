@@ -336,53 +336,136 @@ __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, ui
 
 // ---------------------------------------------------------------------------
 // Warm start: the surrogate's normalized fields q [m][m][3] =
-// ((rho - 1) / U^2, ux / U, uy / U) on cell centres of an m x m grid covering
-// the cavity -> populations f (FP64, reference layout 9 x nx x ny) on nx x ny
-// nodes.  Bilinear interpolation at node centres ((k + 1/2) m / nx - 1/2,
-// clamped to the outermost cell centres), then f = f_eq(rho, u) in the opt1
-// order; with neq, plus the first-order non-equilibrium part of the BGK
-// post-collision state, (1 - omega) * (-3 w_i rho / omega) Q_i : S, with the
-// strain rate S from central differences of the interpolated velocity
-// (one-sided at the walls).
+// ((rho - 1) / U^2, ux / U, uy / U), averages over the cells of an m x m grid
+// covering the cavity -> populations f (FP64, reference layout 9 x nx x ny) on
+// nx x ny nodes.  Node (k, l) sits at cell coordinate (px, py) =
+// ((k + 1/2) m / nx - 1/2, (l + 1/2) m / ny - 1/2); the walls lie on the faces
+// px = -1/2, px = m - 1/2, py = -1/2 and the lid on py = m - 1/2.
+// Reconstruction of the node values:
+//   * default: bilinear between cell centres, clamped beyond the outermost;
+//   * WALLS: bilinear, with ghost cells mirrored across each wall beyond the
+//     outermost centres: velocity ghosts 2 u_wall - u_mirror (u_wall = 0 on the
+//     walls, (1, 0) on the lid, in units of U), density ghosts equal to the
+//     mirror cell, so profiles run into the wall values;
+//   * CUBIC (with the same ghosts, any depth): the cell averages are first
+//     turned into point values at the centres, u_c = q - c * lap(q) (5-point
+//     Laplacian; an r x r node average of a smooth field is its centre value
+//     plus c * lap with c = (r^2 - 1) / (24 r^2)), then interpolated with
+//     separable Catmull-Rom cubics (exact for quadratics);
+// then f = f_eq(rho, u) in the opt1 order; with NEQ, plus the first-order
+// non-equilibrium part of the BGK post-collision state,
+// (1 - omega) * (-3 w_i rho / omega) Q_i : S, with the strain rate S from
+// central differences of the reconstructed velocity (one-sided at the walls).
+constexpr int kWarmNeq = 1, kWarmWalls = 2, kWarmCubic = 4;
 
-__device__ __forceinline__ double bilerp(const float *__restrict__ q, int m, int ch, double px, double py) {
-    px = fmin(fmax(px, 0.0), double(m - 1));
-    py = fmin(fmax(py, 0.0), double(m - 1));
-    int i0 = min(int(px), m - 2 < 0 ? 0 : m - 2), j0 = min(int(py), m - 2 < 0 ? 0 : m - 2);
-    if (m == 1) i0 = j0 = 0;
-    const int i1 = min(i0 + 1, m - 1), j1 = min(j0 + 1, m - 1);
+// q at cell (i, j); cells outside [0, m) are ghosts mirrored across the wall
+// faces (requires i, j in [-m, 2m)).
+__device__ __forceinline__ double cell(const float *__restrict__ q, int m, int ch, int i, int j) {
+    const int ii = i < 0 ? -1 - i : (i >= m ? 2 * m - 1 - i : i);
+    const int jj = j < 0 ? -1 - j : (j >= m ? 2 * m - 1 - j : j);
+    double v = q[(int64_t(ii) * m + jj) * 3 + ch];
+    if (ch == 0) return v;
+    if (j != jj) v = ((j >= m && ch == 1) ? 2.0 : 0.0) - v;  // bottom wall / lid
+    if (i != ii) v = -v;                                   // left / right wall
+    return v;
+}
+
+__device__ __forceinline__ double bilerp(const float *__restrict__ q, int m, int ch, double px,
+                                         double py, bool walls) {
+    int i0, j0, i1, j1;
+    if (walls) {
+        px = fmin(fmax(px, -1.0), double(m));
+        py = fmin(fmax(py, -1.0), double(m));
+        i0 = min(int(floor(px)), m - 1);
+        j0 = min(int(floor(py)), m - 1);
+        i1 = i0 + 1;
+        j1 = j0 + 1;
+    } else {
+        px = fmin(fmax(px, 0.0), double(m - 1));
+        py = fmin(fmax(py, 0.0), double(m - 1));
+        i0 = m < 2 ? 0 : min(int(px), m - 2);
+        j0 = m < 2 ? 0 : min(int(py), m - 2);
+        i1 = min(i0 + 1, m - 1);
+        j1 = min(j0 + 1, m - 1);
+    }
     const double tx = px - i0, ty = py - j0;
-    const double a = q[(int64_t(i0) * m + j0) * 3 + ch], b = q[(int64_t(i1) * m + j0) * 3 + ch];
-    const double c = q[(int64_t(i0) * m + j1) * 3 + ch], d = q[(int64_t(i1) * m + j1) * 3 + ch];
+    const double a = cell(q, m, ch, i0, j0), b = cell(q, m, ch, i1, j0);
+    const double c = cell(q, m, ch, i0, j1), d = cell(q, m, ch, i1, j1);
     return (1 - tx) * ((1 - ty) * a + ty * c) + tx * ((1 - ty) * b + ty * d);
 }
 
+__device__ __forceinline__ void catmull_rom(double t, double w[4]) {
+    const double t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5 * (-t3 + 2.0 * t2 - t);
+    w[1] = 0.5 * (3.0 * t3 - 5.0 * t2 + 2.0);
+    w[2] = 0.5 * (-3.0 * t3 + 4.0 * t2 + t);
+    w[3] = 0.5 * (t3 - t2);
+}
+
+// Point value at the centre of cell (i, j), i, j in [-2, m + 1].
+__device__ __forceinline__ double point(const float *__restrict__ q, int m, int ch, int i, int j,
+                                        double c) {
+    const double v = cell(q, m, ch, i, j);
+    const double lap = (cell(q, m, ch, i + 1, j) + cell(q, m, ch, i - 1, j)) +
+                       (cell(q, m, ch, i, j + 1) + cell(q, m, ch, i, j - 1)) - 4.0 * v;
+    return v - c * lap;
+}
+
+__device__ __forceinline__ double cubic(const float *__restrict__ q, int m, int ch, double px,
+                                       double py, double c) {
+    px = fmin(fmax(px, -0.5), double(m) - 0.5);
+    py = fmin(fmax(py, -0.5), double(m) - 0.5);
+    const int i0 = min(int(floor(px)), m - 1), j0 = min(int(floor(py)), m - 1);
+    double wx[4], wy[4];
+    catmull_rom(px - i0, wx);
+    catmull_rom(py - j0, wy);
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double r = 0.0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) r += wy[b] * point(q, m, ch, i0 - 1 + a, j0 - 1 + b, c);
+        s += wx[a] * r;
+    }
+    return s;
+}
+
+struct Recon {
+    int mode;  // 0 clamped bilinear, 1 wall-ghost bilinear, 2 cubic
+    double c;  // deconvolution coefficient (cubic)
+};
+
+__device__ __forceinline__ double reconstruct(const float *q, int m, int ch, double px, double py,
+                                              Recon r) {
+    return r.mode == 2 ? cubic(q, m, ch, px, py, r.c) : bilerp(q, m, ch, px, py, r.mode == 1);
+}
+
 __device__ __forceinline__ void sample(const float *q, int m, int nx, int ny, int k, int l, double U,
-                                       double &rho, double &ux, double &uy) {
+                                       Recon rc, double &rho, double &ux, double &uy) {
     const double px = (k + 0.5) * m / nx - 0.5, py = (l + 0.5) * m / ny - 0.5;
-    rho = 1.0 + U * U * bilerp(q, m, 0, px, py);
-    ux = U * bilerp(q, m, 1, px, py);
-    uy = U * bilerp(q, m, 2, px, py);
+    rho = 1.0 + U * U * reconstruct(q, m, 0, px, py, rc);
+    ux = U * reconstruct(q, m, 1, px, py, rc);
+    uy = U * reconstruct(q, m, 2, px, py, rc);
 }
 
 __global__ __launch_bounds__(kBlock) void k_warm_init(const float *__restrict__ q, int m, int nx, int ny,
-                                                      double U, double omega, int neq,
+                                                      double U, double omega, int neq, Recon rc,
                                                       double *__restrict__ f) {
     const int64_t j = int64_t(blockIdx.x) * kBlock + threadIdx.x;
     const int64_t P = int64_t(nx) * ny;
     if (j >= P) return;
     const int k = int(j / ny), l = int(j - int64_t(k) * ny);
     double rho, ux, uy;
-    sample(q, m, nx, ny, k, l, U, rho, ux, uy);
+    sample(q, m, nx, ny, k, l, U, rc, rho, ux, uy);
     double feq[9];
     equilibrium_opt1(rho, ux, uy, feq);
     if (neq) {
         const int km = max(k - 1, 0), kp = min(k + 1, nx - 1), lm = max(l - 1, 0), lp = min(l + 1, ny - 1);
         double r_, uxm, uym, uxp, uyp, uxl, uyl, uxu, uyu;
-        sample(q, m, nx, ny, km, l, U, r_, uxm, uym);
-        sample(q, m, nx, ny, kp, l, U, r_, uxp, uyp);
-        sample(q, m, nx, ny, k, lm, U, r_, uxl, uyl);
-        sample(q, m, nx, ny, k, lp, U, r_, uxu, uyu);
+        sample(q, m, nx, ny, km, l, U, rc, r_, uxm, uym);
+        sample(q, m, nx, ny, kp, l, U, rc, r_, uxp, uyp);
+        sample(q, m, nx, ny, k, lm, U, rc, r_, uxl, uyl);
+        sample(q, m, nx, ny, k, lp, U, rc, r_, uxu, uyu);
         const double hx = double(max(kp - km, 1)), hy = double(max(lp - lm, 1));
         const double dxux = (uxp - uxm) / hx, dxuy = (uyp - uym) / hx;
         const double dyux = (uxu - uxl) / hy, dyuy = (uyu - uyl) / hy;
@@ -540,13 +623,18 @@ int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream) {
 }
 
 int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, double omega,
-                      int neq, double *f, void *stream) {
+                      int flags, int pool, double *f, void *stream) {
     if (!q || !f) return LBM_E_NULL;
     if (m <= 0 || !valid_grid(nx, ny)) return LBM_E_SIZE;
-    if (!(U > 0.0) || !(omega > 0.0 && omega < 2.0)) return LBM_E_ARG;
+    if (!(U > 0.0) || !(omega > 0.0 && omega < 2.0) ||
+        (flags & ~(kWarmNeq | kWarmWalls | kWarmCubic)) || pool < 1)
+        return LBM_E_ARG;
+    if ((flags & kWarmCubic) && m < 3) return LBM_E_SIZE;
+    Recon rc{(flags & kWarmCubic) ? 2 : ((flags & kWarmWalls) ? 1 : 0),
+             (double(pool) * pool - 1.0) / (24.0 * double(pool) * pool)};
     const int64_t P = nx * ny;
     hipLaunchKernelGGL(k_warm_init, dim3(blocks_for(P, kBlock)), dim3(kBlock), 0, as_stream(stream), q,
-                       m, int(nx), int(ny), U, omega, neq, f);
+                       m, int(nx), int(ny), U, omega, int((flags & kWarmNeq) != 0), rc, f);
     return launch_status();
 }
 

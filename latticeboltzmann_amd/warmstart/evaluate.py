@@ -33,8 +33,8 @@ from .solve import DEFAULT_WEIGHTS, CavityStepper, Predictor
 class _Fixed:
     """Predictor stand-in returning a fixed field (the pooled cold result)."""
 
-    def __init__(self, q):
-        self.q = q
+    def __init__(self, q, pool=2):
+        self.q, self.pool = q, pool
 
     def predict(self, re, u):
         return self.q
@@ -45,7 +45,7 @@ def field_difference(a, b, U):
     return float(d.max()) / U, float(torch.sqrt((d * d).mean())) / U
 
 
-def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False):
+def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic"):
     rec = {"n": n, "re": re, "u": u}
     with CavityStepper(n, re, u) as st:
         cold = ctl.run(st, warm=False)
@@ -53,9 +53,10 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False):
     rec["cold"] = {k: cold[k] for k in ("steps", "seconds", "reason")}
     legs = [("warm", pred)]
     if pooled:
-        legs.append(("pooled", _Fixed(pooled_target(*fc, u, pred.grid if pred else 128))))
+        m = pred.grid if pred else 128
+        legs.append(("pooled", _Fixed(pooled_target(*fc, u, m), n // m)))
     for name, p in legs:
-        with CavityStepper(n, re, u, predictor=p, neq=neq) as st:
+        with CavityStepper(n, re, u, predictor=p, neq=neq, recon=recon) as st:
             r = ctl.run(st, warm=True)
             fw = st.fields()
             dmax, drms = field_difference(fw, fc, u)
@@ -95,6 +96,7 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-neq", action="store_true")
     ap.add_argument("--pooled", action="store_true")
+    ap.add_argument("--recon", default="cubic", choices=("bilinear", "walls", "cubic"))
     ap.add_argument("--time-limit", type=float, default=None)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
@@ -104,7 +106,7 @@ def main(argv=None):
     for i, (re, u) in enumerate(draw_cavities(a.count, a.seed, pred.re_range, pred.u_range)):
         if a.time_limit and time.time() - t0 > a.time_limit:
             break
-        rec = evaluate_one(pred, a.n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled)
+        rec = evaluate_one(pred, a.n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled, recon=a.recon)
         records.append(rec)
         print(json.dumps(rec), flush=True)
     summary = summarize(records, ("warm", "pooled") if a.pooled else ("warm",))

@@ -25,20 +25,25 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surr
 class Predictor:
     """A trained surrogate plus the normalization it was trained with."""
 
-    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE):
-        self.net, self.grid = net, int(grid)
+    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, pool=2):
+        self.net, self.grid, self.pool = net, int(grid), int(pool)
         self.re_range, self.u_range = tuple(re_range), tuple(u_range)
+
+    @classmethod
+    def from_state(cls, state, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, device=None, pool=2):
+        return cls(Surrogate.from_state(state, device=device), grid, re_range, u_range, pool)
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS, device=None):
         with np.load(path, allow_pickle=False) as z:
             state = {k: z[k] for k in z.files}
-        net = Surrogate.from_state(state, device=device)
-        return cls(net, int(state["grid"]), tuple(state["re_range"]), tuple(state["u_range"]))
+        return cls.from_state(state, int(state["grid"]), tuple(state["re_range"]),
+                              tuple(state["u_range"]), device, int(state.get("pool", 2)))
 
     def save(self, path):
         state = self.net.state_dict()
-        state.update(grid=self.grid, re_range=np.array(self.re_range), u_range=np.array(self.u_range))
+        state.update(grid=self.grid, re_range=np.array(self.re_range), u_range=np.array(self.u_range),
+                     pool=self.pool)
         np.savez(path, **state)
 
     def inputs(self, params):
@@ -51,14 +56,28 @@ class Predictor:
         return self.net.forward(self.inputs([(re, u)]))[0].contiguous()
 
 
-def warm_populations(q, nx, ny, U, omega, neq=True):
-    """FP64 populations (9, nx, ny) on q's device from a prediction q (m, m, 3)."""
+RECONSTRUCTIONS = ("bilinear", "walls", "cubic")
+
+
+def warm_populations(q, nx, ny, U, omega, neq=True, recon="cubic", pool=2):
+    """FP64 populations (9, nx, ny) on q's device from a prediction q (m, m, 3)
+    of cell averages over pool x pool nodes.
+
+    ``recon``: "bilinear" (clamped at the outermost cell centres), "walls"
+    (bilinear towards ghost cells mirrored across the walls, so boundary
+    layers run into the wall and lid velocities) or "cubic" (the same ghosts;
+    cell averages deconvolved to centre values, then Catmull-Rom cubics).
+    ``neq`` adds the post-collision non-equilibrium part of the strain rate."""
+    if recon not in RECONSTRUCTIONS:
+        raise ValueError("recon must be one of %s" % (RECONSTRUCTIONS,))
     q = q.contiguous().to(torch.float32)
     m = q.shape[0]
     if q.shape != (m, m, 3):
         raise ValueError("prediction must have shape (m, m, 3)")
     f = torch.empty((9, nx, ny), dtype=torch.float64, device=q.device)
-    nat.call("lbm_warm_init_f64", q.data_ptr(), m, nx, ny, float(U), float(omega), int(bool(neq)),
+    flags = (nat.WARM_NEQ if neq else 0) | {"bilinear": 0, "walls": nat.WARM_WALLS,
+                                            "cubic": nat.WARM_CUBIC}[recon]
+    nat.call("lbm_warm_init_f64", q.data_ptr(), m, nx, ny, float(U), float(omega), flags, int(pool),
              f.data_ptr(), nat.stream_ptr())
     return f
 
@@ -66,11 +85,12 @@ def warm_populations(q, nx, ny, U, omega, neq=True):
 class CavityStepper:
     """FP64 N x N cavity at (Re, U) for the ``Controller``; use as a context manager."""
 
-    def __init__(self, n, re, u, predictor=None, neq=True, every=1000, graph=True,
+    def __init__(self, n, re, u, predictor=None, neq=True, recon="cubic", every=1000, graph=True,
                  variant="auto", device=None):
         self.n, self.re, self.u = int(n), float(re), float(u)
         self.omega = omega_for_cavity(self.n, self.re, self.u)
-        self.predictor, self.neq, self.every = predictor, bool(neq), int(every)
+        self.predictor, self.neq, self.recon = predictor, bool(neq), recon
+        self.every = int(every)
         self.lat = Lattice(self.n, self.n, self.omega, "cavity", self.u, torch.float64,
                            device=device, variant=variant)
         self._graph = StepGraph(self.lat, self.every, enabled=graph)
@@ -95,7 +115,8 @@ class CavityStepper:
         if self.predictor is None:
             raise RuntimeError("a warm start needs a predictor")
         q = self.predictor.predict(self.re, self.u)
-        self.lat.load(warm_populations(q, self.n, self.n, self.u, self.omega, self.neq))
+        pool = getattr(self.predictor, "pool", 2)
+        self.lat.load(warm_populations(q, self.n, self.n, self.u, self.omega, self.neq, self.recon, pool))
         self.lat.time = 0
 
     def advance(self, n):

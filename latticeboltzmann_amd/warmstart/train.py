@@ -1,8 +1,13 @@
 """Train the warm-start surrogate on the GPU with the project's own kernels.
 
 Adam with a linear warm-up and cosine decay of the learning rate, MSE on the
-normalized fields, mini-batches drawn without replacement each epoch; a
-held-out fraction of the samples gives the validation loss.
+targets standardized per channel (the density channel (rho - 1)/U^2 peaks at
+the lid corners and would otherwise dominate the velocity channels),
+mini-batches drawn without replacement each epoch; a held-out fraction of the
+samples gives the validation loss.  The saved network predicts the physical
+normalized fields directly: the standardization is folded into the linear
+head (rows of its weights times the channel's scale, bias times the scale plus
+the channel's mean).
 
 Usage::
 
@@ -39,6 +44,27 @@ def lr_at(step, total, lr, warmup=100, floor=0.02):
     return lr * (floor + (1 - floor) * 0.5 * (1 + math.cos(math.pi * min(t, 1.0))))
 
 
+def folded_state(net, mean, std):
+    """State of ``net`` with y = std * y_standardized + mean folded into the head."""
+    state = net.state_dict()
+    flat = state["flat"].copy()
+    a, b = net.offsets["head"]
+    cin, cout, _ = net.spec["head"]
+    flat[a:b] = (flat[a:b].reshape(cout, 9 * cin) * std[:, None]).ravel()
+    flat[b:b + cout] = flat[b:b + cout] * std + mean
+    state["flat"] = flat.astype(np.float32)
+    return state
+
+
+def channel_errors(pred, X, Y, batch):
+    """RMS error per output channel of a (folded) predictor, in the targets' units."""
+    se = torch.zeros(Y.shape[-1], dtype=torch.float64, device=X.device)
+    for a in range(0, X.shape[0], batch):
+        d = (pred.net.forward(X[a:a + batch]) - Y[a:a + batch]).double()
+        se += (d * d).sum(dim=(0, 1, 2))
+    return (se / (Y.numel() // Y.shape[-1])).sqrt().cpu().numpy()
+
+
 def evaluate_loss(net, X, Y, batch):
     tot = torch.zeros(1, dtype=torch.float64, device=X.device)
     for a in range(0, X.shape[0], batch):
@@ -55,12 +81,20 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
     pred = Predictor(Surrogate(widths=widths, mid=mid, seed=seed))
     net = pred.net
     X = pred.inputs(params)
-    Y = torch.as_tensor(targets).to(net.device)
     rng = np.random.default_rng(seed)
     order = rng.permutation(S)
     nval = int(round(val_frac * S))
     val, tr = order[:nval], order[nval:]
+    mean = targets[tr].mean(axis=(0, 1, 2)).astype(np.float64)
+    std = targets[tr].std(axis=(0, 1, 2)).astype(np.float64)
+    Yphys = torch.as_tensor(targets).to(net.device)
+    Y = torch.as_tensor(((targets - mean) / std).astype(np.float32)).to(net.device)
     Xv, Yv = X[torch.as_tensor(val)], Y[torch.as_tensor(val)]
+
+    def save():
+        if out:
+            Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
+                                 pred.u_range).save(out)
     steps_per_epoch = max(1, len(tr) // batch)
     total = epochs * steps_per_epoch
     step, t0, hist = 0, time.time(), []
@@ -78,13 +112,19 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
             hist.append(rec)
             if log:
                 log(json.dumps(rec))
-            if out:
-                pred.save(out)
+            save()
         if time_limit and time.time() - t0 > time_limit:
             break
-    if out:
-        pred.save(out)
-    return pred, hist
+    save()
+    final = Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
+                                 pred.u_range, device=net.device)
+    if nval:
+        rms = channel_errors(final, Xv, Yphys[torch.as_tensor(val)], batch)
+        rec = {"val_rms_rho": float(rms[0]), "val_rms_ux": float(rms[1]), "val_rms_uy": float(rms[2])}
+        hist.append(rec)
+        if log:
+            log(json.dumps(rec))
+    return final, hist
 
 
 def main(argv=None):

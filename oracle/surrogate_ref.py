@@ -8,7 +8,8 @@ code except the equilibrium (``cavity_opt1.py:100-132``, via
 
 * ``conv3x3`` / ``conv3x3_backward``: ``conv3x3`` against a direct four-loop
   sum over taps and channels; the backward against central finite differences
-  of the forward (``network_gradcheck``) and the adjoint identity
+  of the whole network's forward, per layer and along random directions, and
+  the adjoint identity
   <im2col(x), y> = <x, col2im(y)>.
 * ``avgpool2`` / ``upsample2`` and their backwards: adjoint identities and
   hand-computed 2x2 cases.
@@ -18,7 +19,15 @@ code except the equilibrium (``cavity_opt1.py:100-132``, via
   on a quadratic whose minimum it must approach and on the first step, whose
   size is lr * sign(g).
 * ``warm_init``: bilinear interpolation is exact for fields linear in x and
-  y away from the clamped border; constant fields give the equilibrium of
+  y away from the clamped border, and, with wall ghosts, up to the walls for
+  linear profiles that meet the wall and lid velocities (u = 0 on the walls,
+  ux = U on the lid) with zero normal density gradient; the cubic form is
+  exact for quadratics in the interior (the pool x pool node average of a
+  quadratic is its centre value plus (pool^2 - 1)/(24 pool^2) times its
+  Laplacian, which the 5-point stencil gives exactly, and Catmull-Rom
+  reproduces quadratics), and up to the walls for fields linear in the
+  wall-normal distance; constant fields give
+  the equilibrium of
   ``d2q9_numpy``; the non-equilibrium part has zero mass and momentum and
   second moment -(2/3)(1 - omega) rho / omega * S (Chapman-Enskog, D2Q9
   fourth-order isotropy).
@@ -244,6 +253,7 @@ def network_loss(params, x, t, widths):
 # -- warm-start initializer ------------------------------------------------------------
 
 def _bilinear(q, m, px, py):
+    """Clamped bilinear interpolation of one channel q (m, m) at cell coordinates."""
     px = np.clip(px, 0.0, m - 1)
     py = np.clip(py, 0.0, m - 1)
     i0 = np.minimum(np.floor(px).astype(np.int64), max(m - 2, 0))
@@ -256,18 +266,92 @@ def _bilinear(q, m, px, py):
     return (1 - tx) * ((1 - ty) * a + ty * c) + tx * ((1 - ty) * b + ty * d)
 
 
-def sample_fields(q, nx, ny, U):
-    """(rho, ux, uy) at the nodes, q (m, m, 3) on cell centres."""
+def ghost_padded(q, depth=1):
+    """(m+2d, m+2d, 3) array with d rings of ghost cells mirrored across the
+    walls: cell -1-j mirrors cell j, cell m+j mirrors m-1-j; velocity ghost =
+    2 u_wall - u_mirror (u_wall = 0 on the left, right and bottom walls, (1, 0)
+    on the lid y = m - 1/2, in units of U), density ghost = density of the
+    mirror cell.  Corner ghosts apply the y rule, then the x rule."""
+    q = np.asarray(q, dtype=np.float64)
+    m = q.shape[0]
+    idx = np.arange(-depth, m + depth)
+    mir = np.where(idx < 0, -1 - idx, np.where(idx >= m, 2 * m - 1 - idx, idx))
+    g = q[mir][:, mir].copy()
+    bottom, top = idx < 0, idx >= m
+    g[:, bottom, 1:] = -g[:, bottom, 1:]
+    g[:, top, 1] = 2.0 - g[:, top, 1]
+    g[:, top, 2] = -g[:, top, 2]
+    out = (idx < 0) | (idx >= m)
+    g[out, :, 1:] = -g[out, :, 1:]
+    return g
+
+
+def _bilinear_walls(gq, m, px, py):
+    """Bilinear interpolation of one ghost-padded (depth 1) channel gq (m+2, m+2)."""
+    px = np.clip(px, -1.0, m)
+    py = np.clip(py, -1.0, m)
+    i0 = np.minimum(np.floor(px).astype(np.int64), m - 1)
+    j0 = np.minimum(np.floor(py).astype(np.int64), m - 1)
+    tx = px - i0
+    ty = py - j0
+    a, b = gq[i0 + 1, j0 + 1], gq[i0 + 2, j0 + 1]
+    c, d = gq[i0 + 1, j0 + 2], gq[i0 + 2, j0 + 2]
+    return (1 - tx) * ((1 - ty) * a + ty * c) + tx * ((1 - ty) * b + ty * d)
+
+
+def catmull_rom(t):
+    """Weights of cells i0-1 .. i0+2 for a point at i0 + t (Catmull and Rom, 1974)."""
+    t2, t3 = t * t, t * t * t
+    return [0.5 * (-t3 + 2 * t2 - t), 0.5 * (3 * t3 - 5 * t2 + 2), 0.5 * (-3 * t3 + 4 * t2 + t),
+            0.5 * (t3 - t2)]
+
+
+def centre_values(q, pool):
+    """Point values at the cell centres from averages over pool x pool nodes,
+    q - c * lap(q), c = (pool^2 - 1) / (24 pool^2); ghost-padded by 2."""
+    g = ghost_padded(q, 3)
+    v = g[1:-1, 1:-1]
+    lap = (g[2:, 1:-1] + g[:-2, 1:-1]) + (g[1:-1, 2:] + g[1:-1, :-2]) - 4.0 * v
+    c = (pool * pool - 1.0) / (24.0 * pool * pool)
+    return v - c * lap
+
+
+def _cubic(pc, m, px, py):
+    """Separable Catmull-Rom interpolation of centre values pc (padded by 2)."""
+    px = np.clip(px, -0.5, m - 0.5)
+    py = np.clip(py, -0.5, m - 0.5)
+    i0 = np.minimum(np.floor(px).astype(np.int64), m - 1)
+    j0 = np.minimum(np.floor(py).astype(np.int64), m - 1)
+    wx, wy = catmull_rom(px - i0), catmull_rom(py - j0)
+    s = 0.0
+    for a in range(4):
+        r = 0.0
+        for b in range(4):
+            r = r + wy[b] * pc[i0 - 1 + a + 2, j0 - 1 + b + 2]
+        s = s + wx[a] * r
+    return s
+
+
+def sample_fields(q, nx, ny, U, recon="bilinear", pool=2):
+    """(rho, ux, uy) at the nodes from cell averages q (m, m, 3).
+
+    recon: "bilinear" (clamped), "walls" (bilinear with mirrored ghosts) or
+    "cubic" (ghosts, deconvolution to centre values, Catmull-Rom)."""
     q = np.asarray(q, dtype=np.float64)
     m = q.shape[0]
     k = np.arange(nx, dtype=np.float64)[:, None]
     l = np.arange(ny, dtype=np.float64)[None, :]
     px = np.broadcast_to((k + 0.5) * m / nx - 0.5, (nx, ny))
     py = np.broadcast_to((l + 0.5) * m / ny - 0.5, (nx, ny))
-    rho = 1.0 + U * U * _bilinear(q[..., 0], m, px, py)
-    ux = U * _bilinear(q[..., 1], m, px, py)
-    uy = U * _bilinear(q[..., 2], m, px, py)
-    return rho, ux, uy
+    if recon == "walls":
+        g = ghost_padded(q)
+        ch = [_bilinear_walls(g[..., c], m, px, py) for c in range(3)]
+    elif recon == "cubic":
+        pc = centre_values(q, pool)
+        ch = [_cubic(pc[..., c], m, px, py) for c in range(3)]
+    else:
+        ch = [_bilinear(q[..., c], m, px, py) for c in range(3)]
+    return 1.0 + U * U * ch[0], U * ch[1], U * ch[2]
 
 
 def strain_rate(ux, uy):
@@ -298,8 +382,8 @@ def neq_post_collision(rho, sxx, sxy, syy, omega):
     return out
 
 
-def warm_init(q, nx, ny, U, omega, neq):
-    rho, ux, uy = sample_fields(q, nx, ny, U)
+def warm_init(q, nx, ny, U, omega, neq, recon="bilinear", pool=2):
+    rho, ux, uy = sample_fields(q, nx, ny, U, recon, pool)
     f = o.equilibrium(rho, ux, uy)
     if neq:
         f = f + neq_post_collision(rho, *strain_rate(ux, uy), omega)

@@ -261,15 +261,19 @@ def test_training_reduces_loss_and_is_deterministic(gpu):
     assert torch.equal(finals[0], finals[1])
 
 
+@pytest.mark.parametrize("recon", ["bilinear", "walls", "cubic"])
 @pytest.mark.parametrize("neq", [False, True])
-def test_warm_init_matches_oracle(gpu, neq):
+def test_warm_init_matches_oracle(gpu, neq, recon):
     from latticeboltzmann_amd.warmstart.solve import warm_populations
     rng = np.random.default_rng(11)
-    m, nx, ny, U, om = 16, 40, 24, 0.08, 1.3
-    q = rng.normal(size=(m, m, 3)).astype(np.float32)
-    f = warm_populations(torch.as_tensor(q).cuda(), nx, ny, U, om, neq=neq)
-    ref = s.warm_init(q.astype(np.float64), nx, ny, U, om, neq)
-    np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-14, atol=1e-16)
+    for m, nx, ny, pool in ((16, 40, 24, 2), (8, 64, 64, 4), (3, 5, 7, 1), (1, 5, 7, 2)):
+        if recon == "cubic" and m < 3:
+            continue
+        U, om = 0.08, 1.3
+        q = rng.normal(size=(m, m, 3)).astype(np.float32)
+        f = warm_populations(torch.as_tensor(q).cuda(), nx, ny, U, om, neq=neq, recon=recon, pool=pool)
+        ref = s.warm_init(q.astype(np.float64), nx, ny, U, om, neq, recon, pool)
+        np.testing.assert_allclose(f.cpu().numpy(), ref, rtol=1e-13, atol=1e-15)
 
 
 def test_warm_init_rejects_bad_arguments(gpu):
@@ -281,6 +285,14 @@ def test_warm_init_rejects_bad_arguments(gpu):
         warm_populations(q, 8, 8, 0.1, 2.0)
     with pytest.raises(ValueError):
         warm_populations(q[:, :2], 8, 8, 0.1, 1.0)
+    with pytest.raises(ValueError):
+        warm_populations(q, 8, 8, 0.1, 1.0, recon="spline")
+    nat = _nat()
+    lib = nat.load()
+    f = torch.empty((9, 8, 8), dtype=torch.float64, device="cuda")
+    assert lib.lbm_warm_init_f64(_p(q), 4, 8, 8, 0.1, 1.0, 8, 2, _p(f), nat.stream_ptr()) == -3
+    assert lib.lbm_warm_init_f64(_p(q), 4, 8, 8, 0.1, 1.0, 0, 0, _p(f), nat.stream_ptr()) == -3
+    assert lib.lbm_warm_init_f64(_p(q), 2, 8, 8, 0.1, 1.0, 4, 2, _p(f), nat.stream_ptr()) == -2
 
 
 class _Fixed:

@@ -191,6 +191,107 @@ def test_bilinear_exact_for_linear_fields():
     np.testing.assert_allclose(uy / U, exact[..., 2], atol=1e-12)
 
 
+def test_wall_ghosts_exact_for_profiles_meeting_the_walls():
+    # Cell averages of fields linear in x and y that vanish on the walls
+    # x = -1/2, y = -1/2 (cell units) are reproduced exactly at every node,
+    # including the half cell next to each wall, where clamping is wrong.
+    m, nx, ny, U = 8, 32, 24, 0.07
+    i = np.arange(m, dtype=float)[:, None]
+    j = np.arange(m, dtype=float)[None, :]
+    a, b = 0.03, -0.02
+    q = np.zeros((m, m, 3))
+    q[..., 1] = a * (i + 0.5) * (j + 0.5)   # bilinear, zero on both walls
+    q[..., 2] = b * (i + 0.5) + 0 * j       # zero on the left wall only ...
+    q[..., 0] = 0.4 + 0 * i * j             # uniform density
+    rho, ux, uy = s.sample_fields(q, nx, ny, U, recon="walls")
+    px = (np.arange(nx)[:, None] + 0.5) * m / nx - 0.5
+    py = (np.arange(ny)[None, :] + 0.5) * m / ny - 0.5
+    # ux is exact next to the left and bottom walls (it vanishes on both)
+    near = np.broadcast_to((px <= m - 1) & (py <= m - 1), (nx, ny))
+    np.testing.assert_allclose((ux / U)[near], np.broadcast_to(a * (px + 0.5) * (py + 0.5), (nx, ny))[near],
+                               atol=1e-13)
+    # uy vanishes on the left wall only: exact next to it, away from the
+    # half cells along the bottom wall and the lid (uy = 0 on both)
+    left = np.broadcast_to((px <= m - 1) & (py >= 0) & (py <= m - 1), (nx, ny))
+    np.testing.assert_allclose((uy / U)[left], np.broadcast_to(b * (px + 0.5), (nx, ny))[left], atol=1e-13)
+    assert (px < 0).any() and (py < 0).any()
+    np.testing.assert_allclose(rho, 1 + 0.4 * U * U, rtol=1e-15)
+    # the clamped form misses the wall layer
+    _, ux_c, _ = s.sample_fields(q, nx, ny, U, recon="bilinear")
+    assert np.abs(ux_c[0, 0] / U - a * (px[0, 0] + 0.5) * (py[0, 0] + 0.5)) > 1e-3
+
+
+def test_wall_ghosts_lid_profile():
+    # ux = U (1 - c (m - 1/2 - y)) meets the lid value U at y = m - 1/2 and is
+    # reproduced up to the lid; the mirror rule sets the ghost to 2 U - u
+    m, nx, U, c = 8, 32, 0.1, 0.1
+    j = np.arange(m, dtype=float)
+    q = np.zeros((m, m, 3))
+    q[..., 1] = (1.0 - c * (m - 0.5 - (j + 0.0)))[None, :]
+    g = s.ghost_padded(q)
+    np.testing.assert_allclose(g[1:-1, -1, 1], 2.0 - q[:, -1, 1])
+    assert np.all(g[0, 1:-1, 1] == -q[0, :, 1]) and np.all(g[1:-1, 0, 2] == -q[:, 0, 2])
+    _, ux, _ = s.sample_fields(q, nx, nx, U, recon="walls")
+    py = (np.arange(nx) + 0.5) * m / nx - 0.5
+    top = py >= m - 1
+    np.testing.assert_allclose(ux[nx // 2, top] / U, 1.0 - c * (m - 0.5 - py[top]), atol=1e-13)
+
+
+def _node_average(u, pool):
+    n = u.shape[0]
+    return u.reshape(n // pool, pool, n // pool, pool, *u.shape[2:]).mean(axis=(1, 3))
+
+
+@pytest.mark.parametrize("pool", [1, 2, 4])
+def test_cubic_exact_for_quadratics_in_the_interior(pool):
+    m = 12
+    n = m * pool
+    x = (np.arange(n) + 0.5) / pool - 0.5  # node positions in cell units
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    fields = [0.3 + 0.02 * X * X - 0.01 * X * Y + 0.005 * Y,
+              0.1 * X - 0.03 * Y * Y + 0.02 * X * Y, -0.2 + 0.01 * X * X + 0.04 * Y * Y]
+    true = np.stack(fields, -1)
+    q = _node_average(true, pool)
+    U = 0.1
+    rho, ux, uy = s.sample_fields(q, n, n, U, recon="cubic", pool=pool)
+    got = np.stack([(rho - 1) / U ** 2, ux / U, uy / U], -1)
+    inner = (X >= 2) & (X <= m - 3) & (Y >= 2) & (Y <= m - 3)
+    np.testing.assert_allclose(got[inner], true[inner], atol=1e-12)
+    # without the deconvolution the same reconstruction is off for pool > 1
+    if pool > 1:
+        plain = s._cubic(s.ghost_padded(q, 2)[..., 1], m, X, Y)
+        assert np.abs(plain[inner] - true[..., 1][inner]).max() > 1e-4
+
+
+def test_cubic_exact_up_to_walls_for_linear_wall_profiles():
+    # u vanishing linearly on the left and bottom walls; ux meeting the lid
+    # linearly: ghosts continue these profiles exactly
+    m, pool = 10, 2
+    n = m * pool
+    x = (np.arange(n) + 0.5) / pool - 0.5
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    uy = 0.02 * (X + 0.5) * (Y + 0.5)
+    ux = 1.0 - 0.05 * (m - 0.5 - Y)
+    true = np.stack([0.7 + 0 * X, ux, uy], -1)
+    q = _node_average(true, pool)
+    rho, gx, gy = s.sample_fields(q, n, n, 1.0, recon="cubic", pool=pool)
+    left_bottom = (X <= m / 2) & (Y <= m / 2)
+    np.testing.assert_allclose(gy[left_bottom], uy[left_bottom], atol=1e-12)
+    top = (Y >= m / 2) & (X >= 2) & (X <= m - 3)
+    np.testing.assert_allclose(gx[top], ux[top], atol=1e-12)
+    np.testing.assert_allclose(rho, 1.7, rtol=1e-14)
+
+
+def test_ghost_mirror_indices_deep():
+    q = np.random.default_rng(3).normal(size=(5, 5, 3))
+    g = s.ghost_padded(q, 3)
+    # cell -3 mirrors cell 2, cell m+2 mirrors cell m-3 (x ghosts flip velocity)
+    np.testing.assert_array_equal(g[0, 3:-3, 0], q[2, :, 0])
+    np.testing.assert_array_equal(g[0, 3:-3, 1:], -q[2, :, 1:])
+    np.testing.assert_array_equal(g[-1, 3:-3, 2], -q[5 - 3, :, 2])
+    np.testing.assert_array_equal(g[3:-3, -1, 1], 2.0 - q[:, 2, 1])
+
+
 def test_neq_moments_match_chapman_enskog():
     rng = np.random.default_rng(11)
     rho = 1 + 0.01 * rng.normal(size=(5, 6))
