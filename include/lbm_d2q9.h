@@ -270,8 +270,10 @@ int lbm_nn_col2im3x3(const float *dcols, int B, int H, int W, int C, float *dx, 
                      void *stream);
 /* dy *= (y > 0), with an optional BF16 copy of the result.                   */
 int lbm_nn_relu_backward(const float *y, float *dy, void *dy_bf16, int64_t n, void *stream);
-/* db[n] = sum_m dy[m*N + n] (FP64 accumulation).                             */
-int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, void *stream);
+/* db[n] = sum_m dy[m*N + n], FP64 accumulation in a fixed order; work holds
+ * lbm_nn_bias_grad_workspace(N) bytes.                                       */
+int64_t lbm_nn_bias_grad_workspace(int N);
+int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, double *work, void *stream);
 /* 2x2 average pool x [B][H][W][C] -> y [B][H/2][W/2][C] (+ optional BF16 copy). */
 int lbm_nn_avgpool2(const float *x, int B, int H, int W, int C, float *y, void *yb, void *stream);
 int lbm_nn_avgpool2_backward(const float *dy, int B, int H, int W, int C, float *dx,

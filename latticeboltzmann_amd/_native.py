@@ -89,7 +89,8 @@ SIGNATURES = {
     "lbm_nn_im2col3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp]),
     "lbm_nn_col2im3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
     "lbm_nn_relu_backward": (_i32, [_vp, _vp, _vp, _i64, _vp]),
-    "lbm_nn_bias_grad": (_i32, [_vp, _i64, _i32, _vp, _vp]),
+    "lbm_nn_bias_grad_workspace": (_i64, [_i32]),
+    "lbm_nn_bias_grad": (_i32, [_vp, _i64, _i32, _vp, _vp, _vp]),
     "lbm_nn_avgpool2": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _vp]),
     "lbm_nn_avgpool2_backward": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
     "lbm_nn_upsample2": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp, _i32, _i32, _vp]),

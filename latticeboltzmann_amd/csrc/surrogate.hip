@@ -31,14 +31,21 @@ __device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(ui
 // ---------------------------------------------------------------------------
 // GEMM: C[m][n] = sum_k A(m, k) B(k, n), A(m, k) = A[m*sam + k*sak],
 // B(k, n) = B[k*sbk + n*sbn] (BF16), FP32 accumulation.
-// Block: 256 threads = 4 waves, tile 64 x 64, k-step 32 staged in LDS; each
-// wave owns a 32 x 32 quarter = 2 x 2 MFMA 16x16x32 tiles.
+// Block: 256 threads = 4 waves, tile 64 x 64, k-step 32 staged in LDS as
+// As[m][k] and Bs[n][k]; each wave owns a 32 x 32 quarter = 2 x 2 MFMA
+// 16x16x32 tiles, and reads each operand as 8 consecutive k of one row.
 // MFMA operand layout (checked on gfx950): lane l holds A rows l%16,
 // k = (l/16)*8 + i; B cols l%16, k = (l/16)*8 + i; result register r of lane l
 // is row (l/16)*4 + r, col l%16.
+// Tile loads (per operand, chosen on the host): LOAD_K: k contiguous, each
+// thread loads 8 k of one row with one 16-byte load; LOAD_ROW: the row index
+// contiguous, each thread loads 8 rows of one k; LOAD_SCALAR otherwise.  The
+// vector forms need 16-byte alignment of the base and of the other stride
+// and fall back to bounds-checked scalar loads at partial tiles.
 // Split-K: blockIdx.z handles k in [z*kchunk, (z+1)*kchunk); with more than
 // one split, results go to work[z][m][n] and k_gemm_reduce adds them in order.
-constexpr int TM = 64, TN = 64, TK = 32;
+constexpr int TM = 64, TN = 64, TK = 32, TP = TK + 8;
+constexpr int LOAD_SCALAR = 0, LOAD_K = 1, LOAD_ROW = 2;
 
 struct GemmArgs {
     const uint16_t *A;
@@ -51,11 +58,44 @@ struct GemmArgs {
     const float *bias;
     int M, N, K, kchunk, accumulate, relu;
     float *work;   // split-K partials
+    int mode_a, mode_b;
 };
 
+// tile[row][k] <- X(row0 + row, k0 + k) for row < 64, k < 32; X(r, k) =
+// X[r*s_row + k*s_k], zero outside rows < nrows, k < kend.
+__device__ __forceinline__ void load_tile(uint16_t (*tile)[TP], const uint16_t *__restrict__ X,
+                                          int64_t s_row, int64_t s_k, int row0, int nrows, int k0,
+                                          int kend, int mode, int t) {
+    if (mode == LOAD_ROW) {
+        const int kr = t >> 3, rc = (t & 7) * 8;
+        const int k = k0 + kr, r = row0 + rc;
+        if (k < kend && r + 8 <= nrows) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(X + int64_t(k) * s_k + r);
+            const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+            for (int i = 0; i < 8; ++i) tile[rc + i][kr] = e[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                tile[rc + i][kr] = (k < kend && r + i < nrows) ? X[int64_t(k) * s_k + r + i] : 0;
+        }
+        return;
+    }
+    const int rr = t >> 2, kc = (t & 3) * 8;
+    const int r = row0 + rr, k = k0 + kc;
+    if (mode == LOAD_K && r < nrows && k + 8 <= kend) {
+        *reinterpret_cast<uint4 *>(&tile[rr][kc]) =
+            *reinterpret_cast<const uint4 *>(X + int64_t(r) * s_row + k);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        tile[rr][kc + i] = (r < nrows && k + i < kend) ? X[int64_t(r) * s_row + int64_t(k + i) * s_k] : 0;
+}
+
 __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
-    __shared__ uint16_t As[TM][TK + 8];
-    __shared__ uint16_t Bs[TK][TN + 8];
+    __shared__ __align__(16) uint16_t As[TM][TP];
+    __shared__ __align__(16) uint16_t Bs[TN][TP];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kbeg = blockIdx.z * g.kchunk;
@@ -68,36 +108,16 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        {  // A tile 64 x 32: thread t -> row t/4, k (t%4)*8 .. +8
-            const int r = t >> 2, kk = (t & 3) * 8;
-            const int m = m0 + r;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int k = k0 + kk + i;
-                As[r][kk + i] = (m < g.M && k < kend) ? g.A[int64_t(m) * g.sam + int64_t(k) * g.sak] : 0;
-            }
-        }
-        {  // B tile 32 x 64: thread t -> k t/8, n (t%8)*8 .. +8
-            const int kr = t >> 3, nn = (t & 7) * 8;
-            const int k = k0 + kr;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const int n = n0 + nn + i;
-                Bs[kr][nn + i] = (k < kend && n < g.N) ? g.B[int64_t(k) * g.sbk + int64_t(n) * g.sbn] : 0;
-            }
-        }
+        load_tile(As, g.A, g.sam, g.sak, m0, g.M, k0, kend, g.mode_a, t);
+        load_tile(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, kend, g.mode_b, t);
         __syncthreads();
         bf16x8 a[2], b[2];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                a[i][e] = __builtin_bit_cast(__bf16, As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8 + e]);
+            a[i] = *reinterpret_cast<const bf16x8 *>(&As[wm + i * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
         for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int e = 0; e < 8; ++e)
-                b[j][e] = __builtin_bit_cast(__bf16, Bs[(lane >> 4) * 8 + e][wn + j * 16 + (lane & 15)]);
+            b[j] = *reinterpret_cast<const bf16x8 *>(&Bs[wn + j * 16 + (lane & 15)][(lane >> 4) * 8]);
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -146,23 +166,31 @@ __global__ __launch_bounds__(256) void k_gemm_reduce(GemmArgs g, int splits) {
 }
 
 // im2col for 3x3 same convolution: x [B][H][W][C] BF16 -> cols [B*H*W][9*C].
-__global__ __launch_bounds__(256) void k_im2col(const uint16_t *__restrict__ x, int B, int H, int W,
-                                                int C, uint16_t *__restrict__ cols) {
-    const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
-    const int64_t total = int64_t(B) * H * W * 9 * C;
-    if (idx >= total) return;
-    const int c = int(idx % C);
-    int64_t r = idx / C;
-    const int tap = int(r % 9);
-    r /= 9;  // pixel index
-    const int xw = int(r % W);
-    const int64_t r2 = r / W;
-    const int yh = int(r2 % H);
-    const int b = int(r2 / H);
+// Tap (dy, dx) = (blockIdx.y / 3, blockIdx.y % 3); each thread moves VEC
+// consecutive channels of one pixel (VEC = 8: one 16-byte load and store,
+// used when C % 8 == 0).  Indices are 32-bit: the caller checks B*H*W*C < 2^31.
+template <int VEC>
+__global__ __launch_bounds__(256) void k_im2col(const uint16_t *__restrict__ x, int H, int W, int C,
+                                                uint16_t *__restrict__ cols, int items) {
+    const int idx = blockIdx.x * 256 + threadIdx.x;
+    if (idx >= items) return;  // items = B*H*W*C/VEC
+    const int cv = C / VEC;
+    const int c = (idx % cv) * VEC;
+    const int p = idx / cv;
+    const int xw = p % W, r = p / W;
+    const int yh = r % H, b = r / H;
+    const int tap = blockIdx.y;
     const int yy = yh + tap / 3 - 1, xx = xw + tap % 3 - 1;
-    uint16_t v = 0;
-    if (yy >= 0 && yy < H && xx >= 0 && xx < W) v = x[((int64_t(b) * H + yy) * W + xx) * C + c];
-    cols[idx] = v;
+    const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    const int64_t src = ((int64_t(b) * H + yy) * W + xx) * C + c;
+    const int64_t dst = (int64_t(p) * 9 + tap) * C + c;
+    if constexpr (VEC == 8) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (inside) v = *reinterpret_cast<const uint4 *>(x + src);
+        *reinterpret_cast<uint4 *>(cols + dst) = v;
+    } else {
+        cols[dst] = inside ? x[src] : uint16_t(0);
+    }
 }
 
 // col2im (gather): dx[b][y][x][c] (+)= sum over taps of dcols[pixel(y - dy + 1, x - dx + 1)][tap*C + c].
@@ -197,14 +225,40 @@ __global__ __launch_bounds__(256) void k_relu_backward(const float *__restrict__
     if (dyb) dyb[i] = f2bf(g);
 }
 
-// db[n] = sum_m dy[m][n], one block per column, fixed order.
-__global__ __launch_bounds__(kBlock) void k_bias_grad(const float *__restrict__ dy, int64_t M, int N,
-                                                      float *__restrict__ db) {
-    const int n = blockIdx.x;
+// db[n] = sum_m dy[m][n] in two fixed-order passes.  Pass 1: block b sums
+// rows [b*rows, (b+1)*rows) of columns [c0, c0 + cw), cw = min(N - c0, 256);
+// thread t owns column c0 + t % cw and rows t / cw, + 256 / cw, ... (coalesced
+// across a row), the threads of one column are then added in LDS in order.
+constexpr int kBiasBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_bias_grad_partial(const float *__restrict__ dy, int64_t M,
+                                                           int N, int c0, int64_t rows,
+                                                           double *__restrict__ partial) {
+    __shared__ double red[256];
+    const int cw = min(N - c0, 256), t = threadIdx.x;
+    const int lanes = 256 / cw;  // threads per column
+    const int col = t % cw, r0 = t / cw;
     double s = 0.0;
-    for (int64_t m = threadIdx.x; m < M; m += kBlock) s += double(dy[m * N + n]);
-    const double tot = block_sum(s);
-    if (threadIdx.x == 0) db[n] = float(tot);
+    if (r0 < lanes) {
+        const int64_t beg = int64_t(blockIdx.x) * rows, end = min(M, beg + rows);
+        for (int64_t m = beg + r0; m < end; m += lanes) s += double(dy[m * N + c0 + col]);
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < cw) {
+        double v = 0.0;
+        for (int j = 0; j < lanes; ++j) v += red[j * cw + t];
+        partial[int64_t(blockIdx.x) * N + c0 + t] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_bias_grad_final(const double *__restrict__ partial, int N,
+                                                         int nblocks, float *__restrict__ db) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= N) return;
+    double v = 0.0;
+    for (int b = 0; b < nblocks; ++b) v += partial[int64_t(b) * N + n];
+    db[n] = float(v);
 }
 
 // 2x2 average pool (NHWC): y [B][H/2][W/2][C].
@@ -493,6 +547,18 @@ using namespace lbm::nn;
 
 extern "C" {
 
+}  // extern "C"
+
+// Tile-load form for an operand X(row, k) = X[row*s_row + k*s_k].
+static int load_mode(const void *X, int64_t s_row, int64_t s_k) {
+    const bool aligned = reinterpret_cast<uintptr_t>(X) % 16 == 0;
+    if (aligned && s_k == 1 && s_row % 8 == 0) return LOAD_K;
+    if (aligned && s_row == 1 && s_k % 8 == 0) return LOAD_ROW;
+    return LOAD_SCALAR;
+}
+
+extern "C" {
+
 int64_t lbm_nn_gemm_workspace(int M, int N, int ksplit) {
     return ksplit > 1 ? int64_t(ksplit) * M * N * int64_t(sizeof(float)) : 0;
 }
@@ -508,7 +574,8 @@ int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t 
     kchunk = (kchunk + TK - 1) / TK * TK;
     const int splits = (K + kchunk - 1) / kchunk;
     GemmArgs g{static_cast<const uint16_t *>(A), sam, sak, static_cast<const uint16_t *>(B), sbk, sbn,
-               C, static_cast<uint16_t *>(Cb), ldc, bias, M, N, K, kchunk, accumulate, relu, work};
+               C, static_cast<uint16_t *>(Cb), ldc, bias, M, N, K, kchunk, accumulate, relu, work,
+               load_mode(A, sam, sak), load_mode(B, sbn, sbk)};
     hipStream_t s = as_stream(stream);
     dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
     hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
@@ -520,9 +587,19 @@ int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t 
 int lbm_nn_im2col3x3(const void *x, int B, int H, int W, int C, void *cols, void *stream) {
     if (!x || !cols) return LBM_E_NULL;
     if (B <= 0 || H <= 0 || W <= 0 || C <= 0) return LBM_E_SIZE;
-    const int64_t n = int64_t(B) * H * W * 9 * C;
-    hipLaunchKernelGGL(k_im2col, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream),
-                       static_cast<const uint16_t *>(x), B, H, W, C, static_cast<uint16_t *>(cols));
+    const int64_t n = int64_t(B) * H * W * C;
+    if (n >= (int64_t(1) << 31)) return LBM_E_SIZE;
+    const bool vec = (C % 8 == 0) && (reinterpret_cast<uintptr_t>(x) % 16 == 0) &&
+                     (reinterpret_cast<uintptr_t>(cols) % 16 == 0);
+    const int items = int(vec ? n / 8 : n);
+    dim3 grid(blocks_for(items), 9);
+    hipStream_t s = as_stream(stream);
+    if (vec)
+        hipLaunchKernelGGL(k_im2col<8>, grid, dim3(256), 0, s, static_cast<const uint16_t *>(x), H, W, C,
+                           static_cast<uint16_t *>(cols), items);
+    else
+        hipLaunchKernelGGL(k_im2col<1>, grid, dim3(256), 0, s, static_cast<const uint16_t *>(x), H, W, C,
+                           static_cast<uint16_t *>(cols), items);
     return launch_status();
 }
 
@@ -543,10 +620,17 @@ int lbm_nn_relu_backward(const float *y, float *dy, void *dy_bf16, int64_t n, vo
     return launch_status();
 }
 
-int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, void *stream) {
-    if (!dy || !db) return LBM_E_NULL;
+int64_t lbm_nn_bias_grad_workspace(int N) { return int64_t(kBiasBlocks) * N * int64_t(sizeof(double)); }
+
+int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, double *work, void *stream) {
+    if (!dy || !db || !work) return LBM_E_NULL;
     if (M <= 0 || N <= 0) return LBM_E_SIZE;
-    hipLaunchKernelGGL(k_bias_grad, dim3(N), dim3(kBlock), 0, as_stream(stream), dy, M, N, db);
+    hipStream_t s = as_stream(stream);
+    const int64_t rows = (M + kBiasBlocks - 1) / kBiasBlocks;
+    const int nb = int((M + rows - 1) / rows);
+    for (int c0 = 0; c0 < N; c0 += 256)
+        hipLaunchKernelGGL(k_bias_grad_partial, dim3(nb), dim3(256), 0, s, dy, M, N, c0, rows, work);
+    hipLaunchKernelGGL(k_bias_grad_final, dim3(blocks_for(N)), dim3(256), 0, s, work, N, nb, db);
     return launch_status();
 }
 

@@ -203,7 +203,9 @@ class Surrogate:
         # dW[co][k] = sum_p dy[p][co] cols[p][k]
         nat.call("lbm_nn_gemm", _p(dyb), 1, cout, _p(cols), K, 1, _p(gw), None, K, None,
                  cout, K, P, 0, 0, ks, None if work is None else _p(work), s)
-        nat.call("lbm_nn_bias_grad", _p(dy), P, cout, _p(gbias), s)
+        bw = torch.empty(int(nat.load().lbm_nn_bias_grad_workspace(cout)) // 8, dtype=torch.float64,
+                         device=self.device)
+        nat.call("lbm_nn_bias_grad", _p(dy), P, cout, _p(gbias), _p(bw), s)
         if not need_dx:
             return None
         # dcols[p][k] = sum_co dy[p][co] W[co][k]; dx = col2im(dcols)

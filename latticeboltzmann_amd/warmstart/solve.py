@@ -16,7 +16,7 @@ import torch
 from .. import _native as nat
 from .. import kernels
 from ..lattice import Lattice, StepGraph, omega_for_cavity
-from .features import GRID, RE_RANGE, U_RANGE, features
+from .features import GRID, RE_RANGE, U_RANGE, features, in_channels
 from .network import Surrogate
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_weights.npz")
@@ -25,30 +25,36 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surr
 class Predictor:
     """A trained surrogate plus the normalization it was trained with."""
 
-    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, pool=2):
-        self.net, self.grid, self.pool = net, int(grid), int(pool)
+    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, pool=2, octaves=0):
+        self.net, self.grid, self.pool, self.octaves = net, int(grid), int(pool), int(octaves)
         self.re_range, self.u_range = tuple(re_range), tuple(u_range)
+        if net.in_ch != in_channels(self.octaves):
+            raise ValueError("network takes %d input channels, features give %d"
+                             % (net.in_ch, in_channels(self.octaves)))
 
     @classmethod
-    def from_state(cls, state, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, device=None, pool=2):
-        return cls(Surrogate.from_state(state, device=device), grid, re_range, u_range, pool)
+    def from_state(cls, state, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, device=None, pool=2,
+                   octaves=0):
+        return cls(Surrogate.from_state(state, device=device), grid, re_range, u_range, pool, octaves)
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS, device=None):
         with np.load(path, allow_pickle=False) as z:
             state = {k: z[k] for k in z.files}
         return cls.from_state(state, int(state["grid"]), tuple(state["re_range"]),
-                              tuple(state["u_range"]), device, int(state.get("pool", 2)))
+                              tuple(state["u_range"]), device, int(state.get("pool", 2)),
+                              int(state.get("octaves", 0)))
 
     def save(self, path):
         state = self.net.state_dict()
         state.update(grid=self.grid, re_range=np.array(self.re_range), u_range=np.array(self.u_range),
-                     pool=self.pool)
+                     pool=self.pool, octaves=self.octaves)
         np.savez(path, **state)
 
     def inputs(self, params):
         """(B, m, m, 6) device tensor for a list of (Re, U)."""
-        x = np.stack([features(re, u, self.grid, self.re_range, self.u_range) for re, u in params])
+        x = np.stack([features(re, u, self.grid, self.re_range, self.u_range, self.octaves)
+                      for re, u in params])
         return torch.as_tensor(x).to(self.net.device)
 
     def predict(self, re, u):

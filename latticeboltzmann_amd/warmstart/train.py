@@ -24,17 +24,29 @@ import time
 import numpy as np
 import torch
 
+from .features import in_channels
 from .network import Surrogate
 from .solve import Predictor
 
 
-def load_dataset(paths):
+def load_dataset(paths, mean_free_density=True):
+    """(params [(Re, U)], targets (S, m, m, 3) float32).
+
+    With ``mean_free_density`` each sample's density channel loses its
+    spatial mean.  Scaling every population by one factor maps a steady
+    cavity of this scheme to another (BGK, bounce-back and the lid term
+    6 w rho_lid u0 are all homogeneous of degree one in f), so the mean density
+    carries no information about the flow; here it only records how much
+    mass the lid corners leaked before the run converged."""
     params, targets = [], []
     for p in paths:
         with np.load(p, allow_pickle=False) as z:
             params += list(zip(z["re"].tolist(), z["u"].tolist()))
             targets.append(z["targets"].astype(np.float32))
-    return params, np.concatenate(targets)
+    t = np.concatenate(targets)
+    if mean_free_density:
+        t[..., 0] -= t[..., 0].mean(axis=(1, 2), keepdims=True)
+    return params, t
 
 
 def lr_at(step, total, lr, warmup=100, floor=0.02):
@@ -56,6 +68,19 @@ def folded_state(net, mean, std):
     return state
 
 
+def unfold_state(net, state, mean, std):
+    """Inverse of ``folded_state``: load a saved predictor's weights into ``net``
+    as the network of standardized targets (to continue training it)."""
+    flat = np.asarray(state["flat"], dtype=np.float64).copy()
+    if flat.size != net.nparams:
+        raise ValueError("saved network has %d parameters, this one %d" % (flat.size, net.nparams))
+    a, b = net.offsets["head"]
+    cin, cout, _ = net.spec["head"]
+    flat[a:b] = (flat[a:b].reshape(cout, 9 * cin) / std[:, None]).ravel()
+    flat[b:b + cout] = (flat[b:b + cout] - mean) / std
+    net.load_flat(flat.astype(np.float32))
+
+
 def channel_errors(pred, X, Y, batch):
     """RMS error per output channel of a (folded) predictor, in the targets' units."""
     se = torch.zeros(Y.shape[-1], dtype=torch.float64, device=X.device)
@@ -75,10 +100,13 @@ def evaluate_loss(net, X, Y, batch):
 
 
 def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, widths=(32, 64, 128),
-          mid=128, time_limit=None, log=None):
+          mid=128, time_limit=None, log=None, octaves=0, init=None):
+    """Train from scratch, or continue from the predictor saved at ``init``
+    (same architecture, inputs and data split)."""
     params, targets = load_dataset(paths)
     S = len(params)
-    pred = Predictor(Surrogate(widths=widths, mid=mid, seed=seed))
+    pred = Predictor(Surrogate(in_ch=in_channels(octaves), widths=widths, mid=mid, seed=seed),
+                     octaves=octaves)
     net = pred.net
     X = pred.inputs(params)
     rng = np.random.default_rng(seed)
@@ -90,11 +118,16 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
     Yphys = torch.as_tensor(targets).to(net.device)
     Y = torch.as_tensor(((targets - mean) / std).astype(np.float32)).to(net.device)
     Xv, Yv = X[torch.as_tensor(val)], Y[torch.as_tensor(val)]
+    if init:
+        with np.load(init, allow_pickle=False) as z:
+            if int(z["octaves"]) != octaves:
+                raise ValueError("--init network was trained with %d octaves" % int(z["octaves"]))
+            unfold_state(net, {"flat": z["flat"]}, mean, std)
 
     def save():
         if out:
             Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
-                                 pred.u_range).save(out)
+                                 pred.u_range, octaves=octaves).save(out)
     steps_per_epoch = max(1, len(tr) // batch)
     total = epochs * steps_per_epoch
     step, t0, hist = 0, time.time(), []
@@ -117,7 +150,7 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
             break
     save()
     final = Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
-                                 pred.u_range, device=net.device)
+                                 pred.u_range, device=net.device, octaves=octaves)
     if nval:
         rms = channel_errors(final, Xv, Yphys[torch.as_tensor(val)], batch)
         rec = {"val_rms_rho": float(rms[0]), "val_rms_ux": float(rms[1]), "val_rms_uy": float(rms[2])}
@@ -135,11 +168,13 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--val-frac", type=float, default=0.1)
+    ap.add_argument("--octaves", type=int, default=0, help="sin/cos octaves of the scalar inputs")
+    ap.add_argument("--init", default=None, help="continue training from this saved predictor")
     ap.add_argument("--time-limit", type=float, default=None)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     train(a.data, a.epochs, a.batch, a.lr, a.seed, a.val_frac, a.out, time_limit=a.time_limit,
-          log=lambda s: print(s, flush=True))
+          log=lambda s: print(s, flush=True), octaves=a.octaves, init=a.init)
 
 
 if __name__ == "__main__":

@@ -20,6 +20,14 @@ Contents
     the numpy restatement and as the CPU baseline.
 ``ctypes_ref``
     loader for the C restatement.
+``lowprec``
+    the reduced-precision (scaled deviation) codec and its conserving rounding.
+``surrogate_ref``
+    FP64 numpy forward/backward of the warm-start network (optionally with
+    BF16 rounding where the GPU rounds), Adam, and the warm-start
+    initializer's reconstructions (no reference counterpart).
+``cpu_baseline``
+    the timed CPU leg of ``bench.py``.
 
 Pinning (what checks each function; the tests live in ``tests/test_oracle_*.py``)
 --------------------------------------------------------------------------------

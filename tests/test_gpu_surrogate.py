@@ -103,10 +103,12 @@ def test_gemm_rejects_bad_arguments(gpu):
     assert lib.lbm_nn_gemm(_p(x), 1, 1, _p(x), 1, 1, _p(c), None, 1, None, 1, 1, 64, 0, 0, 2, None, s_) == -5
 
 
-def test_im2col_col2im(gpu):
+@pytest.mark.parametrize("C", [3, 16])
+def test_im2col_col2im(gpu, C):
+    # C = 16 takes the 16-byte vector path, C = 3 the scalar one
     nat = _nat()
     rng = np.random.default_rng(6)
-    B, H, W, C = 2, 5, 7, 3
+    B, H, W = 2, 5, 7
     x, xn = _bf(rng, (B, H, W, C))
     cols = torch.empty((B * H * W, 9 * C), dtype=torch.bfloat16, device="cuda")
     nat.call("lbm_nn_im2col3x3", _p(x), B, H, W, C, _p(cols), nat.stream_ptr())
@@ -168,11 +170,14 @@ def test_relu_bias_mse_adam_kernels(gpu):
     nat.call("lbm_nn_relu_backward", _p(y), _p(dy), _p(dyb), n, st)
     assert torch.equal(dy, torch.where(y > 0, dy0, torch.zeros_like(dy0)))
     assert torch.equal(dyb, dy.to(torch.bfloat16))
-    M, N = 1000, 5
-    g = dy.view(M, N)
-    db = torch.empty(N, dtype=torch.float32, device="cuda")
-    nat.call("lbm_nn_bias_grad", _p(g), M, N, _p(db), st)
-    np.testing.assert_allclose(db.double().cpu().numpy(), g.double().cpu().numpy().sum(0), rtol=1e-6)
+    for M, N in ((1000, 5), (250, 20), (3, 300), (70000, 32)):
+        g = torch.as_tensor(rng.normal(size=(M, N)).astype(np.float32)).cuda()
+        db = torch.empty(N, dtype=torch.float32, device="cuda")
+        bw = torch.empty(nat.load().lbm_nn_bias_grad_workspace(N) // 8, dtype=torch.float64,
+                         device="cuda")
+        nat.call("lbm_nn_bias_grad", _p(g), M, N, _p(db), _p(bw), st)
+        ref = g.double().cpu().numpy().sum(0)
+        assert np.all(np.abs(db.double().cpu().numpy() - ref) <= 1e-6 * (np.abs(ref) + 1.0)), (M, N)
     t = torch.as_tensor(rng.normal(size=n).astype(np.float32)).cuda()
     work = torch.empty(nat.load().lbm_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
     loss = torch.empty(1, dtype=torch.float64, device="cuda")
@@ -361,3 +366,22 @@ def test_residual_of_injected_nan_is_infinite(gpu):
     assert np.isfinite(kernels.velocity_residual(lat.f, ux, uy, layout=lat.layout).item())
     lat.f.view(-1)[12345] = float("nan")
     assert kernels.velocity_residual(lat.f, ux, uy, layout=lat.layout).item() == float("inf")
+
+
+def test_fold_and_unfold_output_standardization(gpu):
+    # the saved network predicts std * y + mean through its head; unfolding
+    # recovers the trained weights
+    from latticeboltzmann_amd.warmstart.network import Surrogate
+    from latticeboltzmann_amd.warmstart.train import folded_state, unfold_state
+    net, _ = _small_net(6)
+    mean, std = np.array([0.5, -0.1, 0.02]), np.array([2.0, 0.3, 0.1])
+    x = torch.as_tensor(np.random.default_rng(12).normal(size=(2, 16, 16, 6)).astype(np.float32)).cuda()
+    raw = net.forward(x).double()
+    folded = Surrogate.from_state(folded_state(net, mean, std))
+    out = folded.forward(x).double()
+    expect = raw * torch.as_tensor(std, device="cuda") + torch.as_tensor(mean, device="cuda")
+    # the folded head's BF16 copy rounds differently from the unfolded one
+    assert float((out - expect).norm() / expect.norm()) < 1e-2
+    back = Surrogate(6, WIDTHS, MID, 3)
+    unfold_state(back, folded_state(net, mean, std), mean, std)
+    assert torch.allclose(back.p, net.p, rtol=1e-6, atol=1e-7)

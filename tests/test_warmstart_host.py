@@ -112,5 +112,32 @@ def test_features_layout():
     assert GRID == 128
 
 
+def test_feature_octaves():
+    from latticeboltzmann_amd.warmstart.features import in_channels
+    x = features(300.0, 0.07, m=4, octaves=2)
+    assert x.shape == (4, 4, in_channels(2)) == (4, 4, 14)
+    a, b = float(x[0, 0, 0]), float(x[0, 0, 1])
+    for j in range(2):
+        w = np.pi * 2 ** j
+        np.testing.assert_allclose(x[1, 2, 6 + 4 * j:10 + 4 * j],
+                                   [np.sin(w * a), np.cos(w * a), np.sin(w * b), np.cos(w * b)],
+                                   rtol=1e-6, atol=1e-6)
+    assert np.array_equal(x[..., :6], features(300.0, 0.07, m=4))
+
+
+def test_dataset_density_is_mean_free(tmp_path):
+    from latticeboltzmann_amd.warmstart.train import load_dataset
+    rng = np.random.default_rng(0)
+    t = rng.normal(size=(3, 4, 4, 3)).astype(np.float16)
+    p = tmp_path / "d.npz"
+    np.savez(p, re=np.array([100.0, 200.0, 300.0]), u=np.array([0.05, 0.06, 0.07]), targets=t)
+    params, y = load_dataset([str(p)])
+    assert params[1] == (200.0, 0.06) and y.dtype == np.float32
+    assert np.abs(y[..., 0].mean(axis=(1, 2))).max() < 1e-6
+    np.testing.assert_array_equal(y[..., 1:], t[..., 1:].astype(np.float32))
+    _, raw = load_dataset([str(p)], mean_free_density=False)
+    np.testing.assert_array_equal(raw, t.astype(np.float32))
+
+
 def test_layer_table_matches_oracle():
     assert unet_layers(6, (32, 64, 128), 128, 3) == s.unet_layers(6, (32, 64, 128), 128, 3)
