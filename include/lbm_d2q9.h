@@ -291,9 +291,16 @@ int lbm_nn_copy_channels(const float *src, int Cs, int soff, float *dst, void *d
  * dy = 2 (y - t) / n.                                                        */
 int lbm_nn_mse(const float *y, const float *t, int64_t n, float *dy, double *work, double *loss,
                void *stream);
+/* Large-scale error term over y, t [B][H][W][C] (H, W multiples of f):
+ * e_bar = mean of (y - t) over each f x f block; *loss = weight * mean(e_bar^2)
+ * (FP64, via work of lbm_reduce_workspace_bytes); dy += its gradient.       */
+int lbm_nn_mse_pooled(const float *y, const float *t, int B, int H, int W, int C, int f,
+                      float weight, float *dy, double *work, double *loss, void *stream);
 /* One Adam update (bias-corrected for step >= 1) of FP32 p, with BF16 copy.  */
 int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int64_t n, float lr,
                 float b1, float b2, float eps, int step, void *stream);
+/* ema = decay * ema + (1 - decay) * p, 0 <= decay < 1 (weight averaging).    */
+int lbm_nn_ema(float *ema, const float *p, int64_t n, float decay, void *stream);
 int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream);
 /* Warm-start populations (FP64, layout f[i][k][l]) from a prediction
  * q[m][m][3] = ((rho-1)/U^2, ux/U, uy/U), averages over the cells of an m x m

@@ -98,7 +98,9 @@ SIGNATURES = {
     "lbm_nn_copy_channels": (_i32, [_vp, _i32, _i32, _vp, _vp, _i32, _i32, _i32, _i64, _i32,
                                     _vp]),
     "lbm_nn_mse": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "lbm_nn_mse_pooled": (_i32, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _f32, _vp, _vp, _vp, _vp]),
     "lbm_nn_adam": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _f32, _f32, _f32, _f32, _i32, _vp]),
+    "lbm_nn_ema": (_i32, [_vp, _vp, _i64, _f32, _vp]),
     "lbm_nn_to_bf16": (_i32, [_vp, _vp, _i64, _vp]),
     "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _i32, _vp, _vp]),
 }

@@ -365,6 +365,44 @@ __global__ __launch_bounds__(kBlock) void k_mse(const float *__restrict__ y, con
     if (threadIdx.x == 0) partial[blockIdx.x] = tot / double(n);
 }
 
+// Large-scale error term: e_bar = mean of (y - t) over each f x f block of
+// cells (per image and channel); loss partials of weight * mean(e_bar^2) and
+// dy += weight * 2 e_bar / (n_blocks f^2) on every cell of the block.  One
+// thread per block: each cell is written by exactly one thread.
+__global__ __launch_bounds__(kBlock) void k_mse_pooled(const float *__restrict__ y,
+                                                       const float *__restrict__ t, int B, int H,
+                                                       int W, int C, int f, float weight,
+                                                       float *__restrict__ dy,
+                                                       double *__restrict__ partial) {
+    const int Hb = H / f, Wb = W / f;
+    const int64_t nb = int64_t(B) * Hb * Wb * C;
+    const double inv = 1.0 / (double(f) * f);
+    double s = 0.0;
+    const int64_t stride = int64_t(gridDim.x) * kBlock;
+    for (int64_t q = int64_t(blockIdx.x) * kBlock + threadIdx.x; q < nb; q += stride) {
+        const int c = int(q % C);
+        int64_t r = q / C;
+        const int xb = int(r % Wb);
+        r /= Wb;
+        const int yb = int(r % Hb);
+        const int b = int(r / Hb);
+        const int64_t base = ((int64_t(b) * H + int64_t(yb) * f) * W + int64_t(xb) * f) * C + c;
+        double e = 0.0;
+        for (int i = 0; i < f; ++i)
+            for (int j = 0; j < f; ++j) {
+                const int64_t o = base + (int64_t(i) * W + j) * C;
+                e += double(y[o]) - double(t[o]);
+            }
+        e *= inv;
+        s += e * e;
+        const float g = float(weight * 2.0 * e * inv / double(nb));
+        for (int i = 0; i < f; ++i)
+            for (int j = 0; j < f; ++j) dy[base + (int64_t(i) * W + j) * C] += g;
+    }
+    const double tot = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot * double(weight) / double(nb);
+}
+
 // Adam on FP32 master weights, BF16 copy for the GEMMs.
 __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float *__restrict__ g,
                                               float *__restrict__ m, float *__restrict__ v,
@@ -380,6 +418,13 @@ __global__ __launch_bounds__(256) void k_adam(float *__restrict__ p, const float
     const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
     p[i] = pi;
     if (pb) pb[i] = f2bf(pi);
+}
+
+// Exponential moving average of the parameters: ema = decay * ema + (1 - decay) * p.
+__global__ __launch_bounds__(256) void k_ema(float *__restrict__ ema, const float *__restrict__ p,
+                                             int64_t n, float decay) {
+    const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (i < n) ema[i] = decay * ema[i] + (1.0f - decay) * p[i];
 }
 
 __global__ __launch_bounds__(256) void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y,
@@ -688,6 +733,17 @@ int lbm_nn_mse(const float *y, const float *t, int64_t n, float *dy, double *wor
     return launch_status();
 }
 
+int lbm_nn_mse_pooled(const float *y, const float *t, int B, int H, int W, int C, int f,
+                      float weight, float *dy, double *work, double *loss, void *stream) {
+    if (!y || !t || !dy || !work || !loss) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || C <= 0 || f <= 0 || H % f || W % f) return LBM_E_SIZE;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_mse_pooled, dim3(kReduceBlocks), dim3(kBlock), 0, s, y, t, B, H, W, C, f,
+                       weight, dy, work);
+    launch_sum_pass2(work, kReduceBlocks, 0.0, loss, s);
+    return launch_status();
+}
+
 int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int64_t n, float lr,
                 float b1, float b2, float eps, int step, void *stream) {
     if (!p || !g || !m || !v) return LBM_E_NULL;
@@ -695,6 +751,14 @@ int lbm_nn_adam(float *p, const float *g, float *m, float *v, void *p_bf16, int6
     const float c1 = 1.0f - powf(b1, float(step)), c2 = 1.0f - powf(b2, float(step));
     hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), p, g, m, v,
                        static_cast<uint16_t *>(p_bf16), n, lr, b1, b2, eps, c1, c2);
+    return launch_status();
+}
+
+int lbm_nn_ema(float *ema, const float *p, int64_t n, float decay, void *stream) {
+    if (!ema || !p) return LBM_E_NULL;
+    if (n <= 0) return LBM_E_SIZE;
+    if (!(decay >= 0.0f && decay < 1.0f)) return LBM_E_ARG;
+    hipLaunchKernelGGL(k_ema, dim3(blocks_for(n)), dim3(256), 0, as_stream(stream), ema, p, n, decay);
     return launch_status();
 }
 

@@ -64,6 +64,7 @@ class Surrogate:
         self.m = torch.zeros(n, **f32)
         self.v = torch.zeros(n, **f32)
         self.step_count = 0
+        self.ema = None
         self.work = torch.empty(int(nat.load().lbm_reduce_workspace_bytes()) // 8,
                                 dtype=torch.float64, device=self.device)
         self.init_params(seed)
@@ -255,14 +256,75 @@ class Surrogate:
                  _p(loss), nat.stream_ptr())
         return loss, dy
 
+    def loss(self, out, target, scales=()):
+        """MSE plus, for each (f, weight) in ``scales``, weight times the MSE of
+        the error averaged over f x f cell blocks (large-scale error).  Returns
+        (loss as a device FP64 scalar, dL/d(out))."""
+        loss, dy = self.mse(out, target)
+        B, H, W, C = out.shape
+        for f, wgt in scales:
+            extra = torch.empty(1, dtype=torch.float64, device=self.device)
+            nat.call("lbm_nn_mse_pooled", _p(out), _p(target.contiguous()), B, H, W, C, int(f),
+                     float(wgt), _p(dy), _p(self.work), _p(extra), nat.stream_ptr())
+            loss = loss + extra
+        return loss, dy
+
     def adam(self, lr, b1=0.9, b2=0.999, eps=1e-8):
         self.step_count += 1
         nat.call("lbm_nn_adam", _p(self.p), _p(self.g), _p(self.m), _p(self.v), _p(self.pb),
                  self.nparams, float(lr), b1, b2, eps, self.step_count, nat.stream_ptr())
 
-    def train_step(self, x, target, lr):
+    def ema_update(self, decay):
+        """Exponential moving average of the FP32 parameters (started at the
+        current parameters on the first call)."""
+        if self.ema is None:
+            self.ema = self.p.clone()
+        nat.call("lbm_nn_ema", _p(self.ema), _p(self.p), self.nparams, float(decay), nat.stream_ptr())
+
+    def train_step(self, x, target, lr, scales=()):
         out, cache = self.forward(x, keep=True)
-        loss, dout = self.mse(out, target)
+        loss, dout = self.loss(out, target, scales)
         self.backward(cache, dout)
         self.adam(lr)
         return loss
+
+
+def _bench(argv=None):
+    """Time forward and training steps on synthetic 128 x 128 inputs:
+    ``python -m latticeboltzmann_amd.warmstart.network [--batch 16] [--steps 10]``."""
+    import argparse
+    import json
+    import time
+    ap = argparse.ArgumentParser(description=_bench.__doc__.split("\n")[0])
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--in-ch", type=int, default=18)
+    a = ap.parse_args(argv)
+    net = Surrogate(in_ch=a.in_ch, seed=0)
+    g = torch.Generator(device=net.device).manual_seed(0)
+    x = torch.randn(a.batch, 128, 128, a.in_ch, device=net.device, generator=g)
+    t = torch.randn(a.batch, 128, 128, 3, device=net.device, generator=g)
+    flops = 0
+    for name, cin, cout, _ in net.layers:
+        lvl = int(name[3]) if name[:3] in ("enc", "dec") else (0 if name == "head" else len(net.widths))
+        flops += 2 * 9 * cin * cout * (128 >> lvl) ** 2 * a.batch
+    for _ in range(3):
+        net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.forward(x)
+    torch.cuda.synchronize()
+    tf = (time.perf_counter() - t0) / a.steps
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        net.train_step(x, t, 1e-4)
+    torch.cuda.synchronize()
+    tt = (time.perf_counter() - t0) / a.steps
+    print(json.dumps({"params": net.nparams, "batch": a.batch, "forward_ms": round(tf * 1e3, 3),
+                      "train_step_ms": round(tt * 1e3, 3), "forward_tflops": round(flops / tf / 1e12, 2),
+                      "train_tflops": round(3 * flops / tt / 1e12, 2)}), flush=True)
+
+
+if __name__ == "__main__":
+    _bench()

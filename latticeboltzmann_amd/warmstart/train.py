@@ -99,10 +99,29 @@ def evaluate_loss(net, X, Y, batch):
     return float(tot.item()) / X.shape[0]
 
 
+def large_scale_error(pred, X, Y, batch, block=32):
+    """RMS over validation samples of the velocity error averaged over
+    block x block cells (units of U): the part of the error that only the
+    slowest modes of the lattice remove."""
+    se, n = 0.0, 0
+    for a in range(0, X.shape[0], batch):
+        d = (pred.net.forward(X[a:a + batch]) - Y[a:a + batch])[..., 1:].double()
+        B, H, W, C = d.shape
+        e = d.reshape(B, H // block, block, W // block, block, C).mean(dim=(2, 4))
+        se += float((e * e).sum().item())
+        n += e.numel()
+    return math.sqrt(se / n)
+
+
 def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, widths=(32, 64, 128),
-          mid=128, time_limit=None, log=None, octaves=0, init=None):
+          mid=128, time_limit=None, log=None, octaves=0, init=None, scales=(), ema=0.0):
     """Train from scratch, or continue from the predictor saved at ``init``
-    (same architecture, inputs and data split)."""
+    (same architecture, inputs and data split).
+
+    With ``ema`` > 0 the saved network is the exponential moving average of
+    the weights (decay ``ema`` per step).  At every log point (10 epochs) the
+    saved candidate is scored on the validation split by its large-scale
+    velocity error, and the best one is kept."""
     params, targets = load_dataset(paths)
     S = len(params)
     pred = Predictor(Surrogate(in_ch=in_channels(octaves), widths=widths, mid=mid, seed=seed),
@@ -118,42 +137,64 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
     Yphys = torch.as_tensor(targets).to(net.device)
     Y = torch.as_tensor(((targets - mean) / std).astype(np.float32)).to(net.device)
     Xv, Yv = X[torch.as_tensor(val)], Y[torch.as_tensor(val)]
+    Yv_phys = Yphys[torch.as_tensor(val)]
     if init:
         with np.load(init, allow_pickle=False) as z:
             if int(z["octaves"]) != octaves:
                 raise ValueError("--init network was trained with %d octaves" % int(z["octaves"]))
             unfold_state(net, {"flat": z["flat"]}, mean, std)
 
-    def save():
-        if out:
-            Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
-                                 pred.u_range, octaves=octaves).save(out)
+    def candidate():
+        """The folded predictor of the current (or averaged) weights."""
+        state = folded_state(net, mean, std)
+        if ema > 0 and net.ema is not None:
+            avg = net.state_dict()
+            avg["flat"] = net.ema.cpu().numpy()
+            state = folded_state(Surrogate.from_state(avg, device=net.device), mean, std)
+        return Predictor.from_state(state, pred.grid, pred.re_range, pred.u_range,
+                                    device=net.device, octaves=octaves)
+
     steps_per_epoch = max(1, len(tr) // batch)
     total = epochs * steps_per_epoch
     step, t0, hist = 0, time.time(), []
+    best, best_err = None, math.inf
     for ep in range(epochs):
         perm = rng.permutation(tr)
         acc = torch.zeros(1, dtype=torch.float64, device=net.device)
         for b in range(steps_per_epoch):
             idx = torch.as_tensor(perm[b * batch:(b + 1) * batch])
-            acc += net.train_step(X[idx], Y[idx], lr_at(step, total, lr))
+            acc += net.train_step(X[idx], Y[idx], lr_at(step, total, lr), scales)
+            if ema > 0:
+                net.ema_update(ema)
             step += 1
-        if ep % 10 == 9 or ep == epochs - 1 or (time_limit and time.time() - t0 > time_limit):
+        last = ep == epochs - 1 or (time_limit and time.time() - t0 > time_limit)
+        if ep % 10 == 9 or last:
             rec = {"epoch": ep + 1, "train_loss": float(acc.item()) / steps_per_epoch,
                    "val_loss": evaluate_loss(net, Xv, Yv, batch) if nval else None,
                    "seconds": round(time.time() - t0, 1)}
+            cand = candidate()
+            if nval:
+                err = large_scale_error(cand, Xv, Yv_phys, batch)
+                rec["val_large_scale"] = err
+                if err < best_err:
+                    best, best_err = cand, err
+                    rec["kept"] = True
+            else:
+                best = cand
             hist.append(rec)
             if log:
                 log(json.dumps(rec))
-            save()
-        if time_limit and time.time() - t0 > time_limit:
+            if out and rec.get("kept", not nval):
+                best.save(out)
+        if last:
             break
-    save()
-    final = Predictor.from_state(folded_state(net, mean, std), pred.grid, pred.re_range,
-                                 pred.u_range, device=net.device, octaves=octaves)
+    final = best if best is not None else candidate()
+    if out:
+        final.save(out)
     if nval:
-        rms = channel_errors(final, Xv, Yphys[torch.as_tensor(val)], batch)
-        rec = {"val_rms_rho": float(rms[0]), "val_rms_ux": float(rms[1]), "val_rms_uy": float(rms[2])}
+        rms = channel_errors(final, Xv, Yv_phys, batch)
+        rec = {"val_rms_rho": float(rms[0]), "val_rms_ux": float(rms[1]), "val_rms_uy": float(rms[2]),
+               "val_large_scale": large_scale_error(final, Xv, Yv_phys, batch)}
         hist.append(rec)
         if log:
             log(json.dumps(rec))
@@ -170,11 +211,16 @@ def main(argv=None):
     ap.add_argument("--val-frac", type=float, default=0.1)
     ap.add_argument("--octaves", type=int, default=0, help="sin/cos octaves of the scalar inputs")
     ap.add_argument("--init", default=None, help="continue training from this saved predictor")
+    ap.add_argument("--scale-loss", action="append", default=[], metavar="F:W",
+                    help="add W times the MSE of the error averaged over F x F cells")
+    ap.add_argument("--ema", type=float, default=0.0,
+                    help="keep an exponential moving average of the weights with this decay per step")
     ap.add_argument("--time-limit", type=float, default=None)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     train(a.data, a.epochs, a.batch, a.lr, a.seed, a.val_frac, a.out, time_limit=a.time_limit,
-          log=lambda s: print(s, flush=True), octaves=a.octaves, init=a.init)
+          log=lambda s: print(s, flush=True), octaves=a.octaves, init=a.init,
+          scales=[(int(v.split(":")[0]), float(v.split(":")[1])) for v in a.scale_loss], ema=a.ema)
 
 
 if __name__ == "__main__":

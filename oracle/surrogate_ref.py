@@ -142,6 +142,16 @@ def mse(y, t):
     return float(np.mean(d * d)), 2.0 * d / d.size
 
 
+def mse_pooled(y, t, f, weight):
+    """weight * mean over f x f blocks of (mean of y - t over the block)^2 and its
+    gradient with respect to y (NHWC)."""
+    B, H, W, C = y.shape
+    e = (y - t).reshape(B, H // f, f, W // f, f, C).mean(axis=(2, 4))
+    loss = weight * float(np.mean(e * e))
+    g = weight * 2.0 * e / e.size / (f * f)
+    return loss, np.repeat(np.repeat(g, f, axis=1), f, axis=2)
+
+
 def adam(p, g, m, v, step, lr, b1=0.9, b2=0.999, eps=1e-8):
     """Kingma and Ba (2015), Algorithm 1; returns updated (p, m, v)."""
     m = b1 * m + (1 - b1) * g

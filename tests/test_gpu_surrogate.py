@@ -385,3 +385,38 @@ def test_fold_and_unfold_output_standardization(gpu):
     back = Surrogate(6, WIDTHS, MID, 3)
     unfold_state(back, folded_state(net, mean, std), mean, std)
     assert torch.allclose(back.p, net.p, rtol=1e-6, atol=1e-7)
+
+
+def test_pooled_error_term_matches_oracle(gpu):
+    nat = _nat()
+    rng = np.random.default_rng(14)
+    B, H, W, C, f, wgt = 2, 16, 24, 3, 8, 3.0
+    y = torch.as_tensor(rng.normal(size=(B, H, W, C)).astype(np.float32)).cuda()
+    t = torch.as_tensor(rng.normal(size=(B, H, W, C)).astype(np.float32)).cuda()
+    dy0 = torch.as_tensor(rng.normal(size=(B, H, W, C)).astype(np.float32)).cuda()
+    dy = dy0.clone()
+    work = torch.empty(nat.load().lbm_reduce_workspace_bytes() // 8, dtype=torch.float64, device="cuda")
+    loss = torch.empty(1, dtype=torch.float64, device="cuda")
+    nat.call("lbm_nn_mse_pooled", _p(y), _p(t), B, H, W, C, f, wgt, _p(dy), _p(work), _p(loss),
+             nat.stream_ptr())
+    rl, rg = s.mse_pooled(y.double().cpu().numpy(), t.double().cpu().numpy(), f, wgt)
+    assert abs(float(loss.item()) - rl) <= 1e-12 * rl
+    # dy0 + g is rounded once to FP32: allow one rounding of the sum
+    want = dy0.double().cpu().numpy() + rg
+    got = dy.double().cpu().numpy()
+    assert np.all(np.abs(got - want) <= 2.0 ** -23 * np.abs(want) + 1e-6 * np.abs(rg) + 1e-12)
+    lib = nat.load()
+    assert lib.lbm_nn_mse_pooled(_p(y), _p(t), B, H, W, C, 5, wgt, _p(dy), _p(work), _p(loss),
+                                 nat.stream_ptr()) == -2
+
+
+def test_parameter_average_kernel(gpu):
+    nat = _nat()
+    rng = np.random.default_rng(15)
+    p = torch.as_tensor(rng.normal(size=1000).astype(np.float32)).cuda()
+    ema = torch.as_tensor(rng.normal(size=1000).astype(np.float32)).cuda()
+    e0 = ema.clone()
+    nat.call("lbm_nn_ema", _p(ema), _p(p), 1000, 0.99, nat.stream_ptr())
+    want = 0.99 * e0.double() + 0.01 * p.double()
+    assert float((ema.double() - want).abs().max()) < 1e-6
+    assert nat.load().lbm_nn_ema(_p(ema), _p(p), 1000, 1.0, nat.stream_ptr()) == -3

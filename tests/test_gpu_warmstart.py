@@ -1,0 +1,73 @@
+"""The shipped, trained surrogate: what it gains over the cold start (and over
+an untrained network of the same shape) on held-out cavities, with the
+cold-started LBM as the judge of the converged field."""
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(n, re, u, predictor, ctl):
+    from latticeboltzmann_amd.warmstart.solve import CavityStepper
+    with CavityStepper(n, re, u, predictor=predictor) as st:
+        rec = ctl.run(st, warm=predictor is not None)
+        fields = tuple(t.clone() for t in st.fields())
+    return rec, fields
+
+
+def _diff(a, b, u):
+    return float(torch.sqrt((a[1] - b[1]) ** 2 + (a[2] - b[2]) ** 2).max().item()) / u
+
+
+def test_shipped_weights_load(gpu):
+    from latticeboltzmann_amd.warmstart.features import in_channels
+    from latticeboltzmann_amd.warmstart.solve import DEFAULT_WEIGHTS, Predictor
+    pred = Predictor.load(DEFAULT_WEIGHTS)
+    assert pred.grid == 128 and pred.pool == 2
+    assert pred.net.in_ch == in_channels(pred.octaves)
+    assert 1.0e6 < pred.net.nparams < 1.4e6
+    q = pred.predict(400.0, 0.08)
+    assert q.shape == (128, 128, 3) and bool(torch.isfinite(q).all())
+    # the lid row drives +x flow; the bottom-wall row is nearly at rest
+    assert float(q[:, -1, 1].mean()) > 0.3 and abs(float(q[:, 0, 1].mean())) < 0.1
+
+
+@pytest.mark.parametrize("re,u", [(250.0, 0.09), (640.0, 0.06)])
+def test_trained_warm_start_beats_cold_start(gpu, re, u):
+    # held-out (Re, U): not drawn by the training seeds 0-2
+    from latticeboltzmann_amd.warmstart.controller import Controller
+    from latticeboltzmann_amd.warmstart.solve import Predictor
+    pred = Predictor.load()
+    ctl = Controller(tol=1e-6, every=1000)
+    cold, fc = _run(256, re, u, None, ctl)
+    warm, fw = _run(256, re, u, pred, ctl)
+    d = _diff(fw, fc, u)
+    print("Re %.0f U %.2f: cold %d steps %.3f s, warm %d steps %.3f s, diff %.1e U"
+          % (re, u, cold["steps"], cold["seconds"], warm["steps"], warm["seconds"], d))
+    assert cold["reason"] == "converged" and not warm["fell_back"]
+    assert warm["steps"] * 2 <= cold["steps"]
+    assert d < 1e-4
+
+
+def test_untrained_network_gains_nothing(gpu):
+    # the same pipeline with an untrained network of the same shape: the
+    # controller either falls back or converges no faster than the trained one
+    from latticeboltzmann_amd.warmstart.controller import Controller
+    from latticeboltzmann_amd.warmstart.network import Surrogate
+    from latticeboltzmann_amd.warmstart.solve import Predictor
+    trained = Predictor.load()
+    raw = Surrogate(in_ch=trained.net.in_ch, widths=trained.net.widths, mid=trained.net.mid,
+                    out_ch=3, seed=7)
+    untrained = Predictor(raw, trained.grid, trained.re_range, trained.u_range, trained.pool,
+                          trained.octaves)
+    ctl = Controller(tol=1e-6, every=1000)
+    re, u = 400.0, 0.08
+    tw, _ = _run(256, re, u, trained, ctl)
+    uw, fu = _run(256, re, u, untrained, ctl)
+    print("trained %d steps; untrained %d steps (fell back: %s, %s)"
+          % (tw["steps"], uw["steps"], uw["fell_back"], uw["reason"]))
+    assert uw["residual"] < 1e-6  # the controller still delivers a converged field
+    assert tw["steps"] < uw["steps"]
+    assert np.isfinite(fu[1].cpu().numpy()).all()

@@ -138,6 +138,18 @@ def test_bf16_rounding_ties_to_even():
     assert s.bf16(-2.5) == -2.5
 
 
+def test_pooled_error_term():
+    rng = np.random.default_rng(13)
+    y, t = rng.normal(size=(2, 8, 12, 3)), rng.normal(size=(2, 8, 12, 3))
+    loss, g = s.mse_pooled(y, t, 4, 2.5)
+    e = s.avgpool2(s.avgpool2(y - t))  # 4 x 4 block means by two 2 x 2 pools
+    assert np.isclose(loss, 2.5 * np.mean(e * e), rtol=1e-13)
+    d = np.zeros_like(y)
+    d[1, 5, 7, 2] = 1e-6
+    fd = (s.mse_pooled(y + d, t, 4, 2.5)[0] - s.mse_pooled(y - d, t, 4, 2.5)[0]) / 2e-6
+    assert np.isclose(fd, g[1, 5, 7, 2], rtol=1e-6)
+
+
 def test_adam_first_step_and_quadratic():
     g = np.array([3.0, -0.2, 1e-3])
     p, m, v = s.adam(np.zeros(3), g, np.zeros(3), np.zeros(3), 1, lr=0.01)
