@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep leg")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
-    ap.add_argument("--warm-count", type=int, default=3)
+    ap.add_argument("--warm-count", type=int, default=6)
     ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--copy-only", action="store_true",
                     help="only time lbm_relayout (a plain copy of the 9*n*n FP64 populations) "

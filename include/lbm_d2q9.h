@@ -167,16 +167,20 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
  * seed.  FP32 storage always rounds to nearest.                               */
 #define LBM_ROUND_NEAREST 0
 #define LBM_ROUND_STOCHASTIC 1
-/* Compute type of the collide: FP32 (the reduced-precision form) or FP64
- * (FP32 storage only; other formats return LBM_E_ARG).  With FP64 compute,
- * FP32 storage is rounded like the 16-bit formats (compensated, nearest or
- * stochastic); with FP32 compute FP32 storage needs no rounding and
- * `rounding` is ignored for it.                                               */
+/* Compute type of the collide: FP32 (the reduced-precision form) or FP64,
+ * for every storage format.  With FP64 compute the stored values are single
+ * roundings of the FP64 results (FP32 storage then rounds like the 16-bit
+ * formats: compensated, nearest or stochastic); with FP32 compute FP32
+ * storage needs no rounding and `rounding` is ignored for it.  A flow that
+ * changes by a few storage ulps per step (a long shear wave) needs FP64
+ * compute with stochastic rounding: an FP32 collide's own rounding errors
+ * repeat every step and bias it.                                            */
 #define LBM_COMPUTE_F32 0
 #define LBM_COMPUTE_F64 1
 /* One fused step src -> dst on stored deviations; ux/uy (nullable, FP32
  * [k][l]) receive the pre-collision velocity.  variant in
- * 0 .. lbm_dev_variant_count()-1 or -1.                                         */
+ * 0 .. lbm_dev_variant_count()-1 or -1.  omega, u0 and scale are FP32 values;
+ * the FP64 collide uses them exactly as given.                               */
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
                  float omega, int bc, float u0, float scale, int compute, int rounding,
                  uint32_t seed, float *ux, float *uy, int variant, void *stream);

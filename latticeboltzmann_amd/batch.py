@@ -72,8 +72,8 @@ class BatchedLattices:
         # FP32 storage defaults to an FP64 collide with unbiased rounding: a
         # slowly decaying wave changes by only a few FP32 ulps per step
         compute = compute or ("f64" if fmt == "f32" else "f32")
-        if compute not in COMPUTE or (compute == "f64" and fmt != "f32"):
-            raise ValueError("compute must be 'f32', or 'f64' with FP32 storage")
+        if compute not in COMPUTE:
+            raise ValueError("compute must be 'f32' or 'f64'")
         self.compute, self.compute_name = COMPUTE[compute], compute
         self.instances = [dict(x) for x in instances]
         self.n = len(self.instances)

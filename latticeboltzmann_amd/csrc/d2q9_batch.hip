@@ -235,13 +235,16 @@ int lbm_batch_step(const void *src, void *dst, int fmt, float scale, const lbm_b
     if (!valid_fmt_b(fmt) || !(scale > 0.0f)) return LBM_E_ARG;
     if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
     if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
-    if (compute == LBM_COMPUTE_F64 && fmt != LBM_FMT_F32) return LBM_E_ARG;
     if (variant < 0 || variant >= kNumBatchShapes) return LBM_E_ARG;
     const BatchArgs a{src, dst, scale, insts, tiles, ntiles, rounding == LBM_ROUND_STOCHASTIC ? 1 : 0,
                       seed, epoch, blocks};
     hipStream_t s = as_stream(stream);
     if (compute == LBM_COMPUTE_F64) {
-        launch_batch_fmt<LBM_FMT_F32, double>(a, variant, s);  // FP32 storage only
+        switch (fmt) {
+            case LBM_FMT_F32: launch_batch_fmt<LBM_FMT_F32, double>(a, variant, s); break;
+            case LBM_FMT_BF16: launch_batch_fmt<LBM_FMT_BF16, double>(a, variant, s); break;
+            default: launch_batch_fmt<LBM_FMT_F16, double>(a, variant, s); break;
+        }
     } else {
         switch (fmt) {
             case LBM_FMT_F32: launch_batch_fmt<LBM_FMT_F32, float>(a, variant, s); break;

@@ -107,6 +107,24 @@ __device__ __forceinline__ float sr_f64_to_f32(double x, uint32_t r29) {
     return float(__builtin_bit_cast(double, b));
 }
 
+// Stochastic rounding of an FP64 value straight to a 16-bit format (an FP64
+// collide must not pass through FP32, whose nearest rounding would repeat the
+// same error every step): 32 random bits are added just below the kept
+// significand (BF16 keeps 7 fraction bits, FP16 10, of FP64's 52), then the
+// dropped bits are cleared, which leaves a value the format holds exactly.
+// Values outside the format's normal range round to nearest.
+template <int FMT>
+__device__ __forceinline__ uint16_t sr_f64_to_16(double x, uint32_t r32) {
+    constexpr int drop = FMT == LBM_FMT_BF16 ? 45 : 42;
+    const double a = fabs(x);
+    const bool normal = FMT == LBM_FMT_BF16 ? (a >= 1.1754943508222875e-38 && a < 3.0e38)
+                                            : (a >= 6.103515625e-05 && a < 65504.0);
+    if (!normal) return Codec<FMT>::enc64(x);
+    uint64_t b = __builtin_bit_cast(uint64_t, x);
+    b = (b + (uint64_t(r32) << (drop - 32))) & ~((uint64_t(1) << drop) - 1);
+    return Codec<FMT>::enc64(__builtin_bit_cast(double, b));
+}
+
 // Weight of channel i in the compute type.
 template <typename CT>
 __host__ __device__ constexpr CT wt(int i) {
@@ -185,11 +203,17 @@ __device__ __forceinline__ typename Codec<FMT>::raw round_one(CT x, Dither &dith
         } else {
             return C::enc64(double(x));
         }
+    } else if constexpr (sizeof(CT) == 8) {
+        // 16-bit formats from an FP64 collide: one rounding of the FP64 value
+        if constexpr (SR) {
+            const uint32_t hi = dither.next16(), lo = dither.next16();
+            return sr_f64_to_16<FMT>(double(x), (hi << 16) | lo);
+        } else {
+            return C::enc64(double(x));
+        }
     } else {
-        // 16-bit formats: an FP64 value passes through FP32 first; its relative
-        // error 2^-24 is far below the 2^-8 .. 2^-11 of the storage format
-        if constexpr (SR) return enc_sr<FMT>(float(x), dither.next16());
-        else return C::enc(float(x));
+        if constexpr (SR) return enc_sr<FMT>(x, dither.next16());
+        else return C::enc(x);
     }
 }
 

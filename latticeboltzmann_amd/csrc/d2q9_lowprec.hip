@@ -211,11 +211,15 @@ void launch_dev_fmt(const void *src, void *dst, int bc, const DevArgs &a, int sh
     }
 }
 
-// FP32 compute serves every format; FP64 compute is built for FP32 storage only.
+// Every storage format with an FP32 or an FP64 collide.
 void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc,
                         const DevArgs &a, int shape, hipStream_t s) {
     if (compute == LBM_COMPUTE_F64) {
-        launch_dev_fmt<LBM_FMT_F32, double>(src, dst, bc, a, shape, s);
+        switch (fmt) {
+            case LBM_FMT_F32: launch_dev_fmt<LBM_FMT_F32, double>(src, dst, bc, a, shape, s); break;
+            case LBM_FMT_BF16: launch_dev_fmt<LBM_FMT_BF16, double>(src, dst, bc, a, shape, s); break;
+            default: launch_dev_fmt<LBM_FMT_F16, double>(src, dst, bc, a, shape, s); break;
+        }
         return;
     }
     switch (fmt) {
@@ -300,7 +304,6 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
     if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0f)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
     if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
-    if (compute == LBM_COMPUTE_F64 && fmt != LBM_FMT_F32) return LBM_E_ARG;
     if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
     if (variant < -1 || variant >= kNumDevShapes) return LBM_E_ARG;
     const int64_t bytes = 9 * nx * ny * fmt_bytes(fmt);

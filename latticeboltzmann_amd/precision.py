@@ -16,12 +16,13 @@ Rounding of the 16-bit formats (``rounding``): ``"stochastic"`` (default) or
 error (see ``lbm_dev_step`` in include/lbm_d2q9.h).  Stochastic rounding is
 seeded by ``seed`` and the step counter, so a run is reproducible.
 
-``compute="f64"`` (FP32 storage only) runs the collide in FP64 and rounds the
-result to FP32 with the same compensated rounding.  With an FP32 collide,
-FP32 rounding is always to nearest and therefore biased for flows that change
-by only a few ulps per step (a long-wavelength shear wave at low viscosity
-then decays 10-15 % too slowly); FP64 compute with stochastic rounding
-removes that bias at the same storage cost.
+``compute="f64"`` runs the collide in FP64 and rounds each result once to the
+storage format with the same compensated rounding.  An FP32 collide's own
+rounding errors repeat every step, so flows that change by only a few ulps
+per step come out biased (a 4096^2 shear wave decays ~5 % too slowly with
+16-bit storage; FP32 storage with an FP32 collide is not rounded at all and
+is biased the same way); FP64 compute with stochastic rounding removes that
+bias at the same storage cost.
 """
 
 import math
@@ -50,8 +51,8 @@ class DeviationLattice:
         nat.require_gpu()
         if fmt not in FORMATS:
             raise ValueError("fmt must be one of %s" % sorted(FORMATS))
-        if compute not in COMPUTE or (compute == "f64" and fmt != "f32"):
-            raise ValueError("compute must be 'f32', or 'f64' with FP32 storage")
+        if compute not in COMPUTE:
+            raise ValueError("compute must be 'f32' or 'f64'")
         self.compute = COMPUTE[compute]
         if rounding not in ROUNDING:
             raise ValueError("rounding must be 'nearest' or 'stochastic'")

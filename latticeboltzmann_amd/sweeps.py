@@ -5,7 +5,11 @@
   and total-mass drift against the FP64 run and against the initial mass.
 * Shear wave (default 4096 x 4096, omega 1, amplitude 0.01, 50000 steps):
   viscosity fitted from the amplitude's decay, A(t) ~ A0 exp(-nu k^2 t),
-  against (1/omega - 1/2)/3, per format including FP64.
+  against (1/omega - 1/2)/3, per format including FP64; the 16-bit formats
+  with the FP32 collide and with an FP64 collide (FP32 storage always uses
+  the FP64 collide here: an FP32 collide stores unrounded FP32 results and
+  shares the bias of the 16-bit FP32 collide).
+* The FP16 cavity is also run with the FP64 collide.
 
 Usage::
 
@@ -50,17 +54,18 @@ def cavity_precision(n=8192, re=1000.0, u=0.1, steps=50000, formats=FORMATS, log
     torch.cuda.empty_cache()
     if log:
         log(json.dumps({"cavity": "f64", **out["f64"]}))
-    for fmt in formats:
-        lat = DeviationLattice(n, n, om, "cavity", u, fmt=fmt, variant="auto")
+    for fmt, compute in [(f, "f32") for f in formats] + [("f16", "f64")]:
+        lat = DeviationLattice(n, n, om, "cavity", u, fmt=fmt, variant="auto", compute=compute)
         _, sec = _timed(lambda: lat.step(steps))
         _, ux, uy = lat.moments()
         rec = velocity_errors(ux, uy, uxr, uyr, u)
         m = lat.mass()
         rec.update(seconds=sec, mlups=n * n * steps / sec / 1e6, mass_drift=m / m0 - 1,
-                   mass_vs_f64=m / m64 - 1)
-        out[fmt] = rec
+                   mass_vs_f64=m / m64 - 1, compute=compute)
+        key = fmt if compute == "f32" else fmt + "_f64_collide"
+        out[key] = rec
         if log:
-            log(json.dumps({"cavity": fmt, **rec}))
+            log(json.dumps({"cavity": key, **rec}))
         del lat, ux, uy
         torch.cuda.empty_cache()
     return out
@@ -97,16 +102,18 @@ def shear_precision(n=4096, omega=1.0, amp=0.01, steps=50000, every=2500, format
     if log:
         log(json.dumps({"shear": "f64", **out["f64"]}))
     for fmt in formats:
-        bat = BatchedLattices([dict(nx=n, ny=n, omega=omega, amplitude=amp, direction="x",
-                                    bc="periodic")], fmt=fmt)
-        (times, amps), sec = _timed(lambda: bat.run(steps, sample_every=every, graph=True))
-        nu = _fit(times, amps[:, 0], n)
-        out[fmt] = {"nu": nu, "nu_rel_err": nu / theory - 1, "seconds": sec,
-                    "compute": bat.compute_name}
-        if log:
-            log(json.dumps({"shear": fmt, **out[fmt]}))
-        del bat
-        torch.cuda.empty_cache()
+        for compute in (("f64",) if fmt == "f32" else ("f32", "f64")):
+            bat = BatchedLattices([dict(nx=n, ny=n, omega=omega, amplitude=amp, direction="x",
+                                        bc="periodic")], fmt=fmt, compute=compute)
+            (times, amps), sec = _timed(lambda: bat.run(steps, sample_every=every, graph=True))
+            nu = _fit(times, amps[:, 0], n)
+            key = fmt if compute == "f32" or fmt == "f32" else fmt + "_f64_collide"
+            out[key] = {"nu": nu, "nu_rel_err": nu / theory - 1, "seconds": sec,
+                        "mlups": n * n * steps / sec / 1e6, "compute": compute}
+            if log:
+                log(json.dumps({"shear": key, **out[key]}))
+            del bat
+            torch.cuda.empty_cache()
     return out
 
 

@@ -84,6 +84,36 @@ def test_one_step_against_fp64_oracle(gpu, fmt, bc, shape, rounding):
             assert (err <= tol).all(), (layout, v, float((err / tol).max()))
 
 
+@pytest.mark.parametrize("fmt", lp.FORMATS)
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+def test_one_step_fp64_collide_against_oracle(gpu, fmt, rounding):
+    # FP64 collide, one rounding to the storage format: within the
+    # compensated-rounding bounds of the FP64 oracle step, in every launch shape
+    # (omega and u0 cross the C ABI as FP32 values, which the FP64 collide
+    # uses exactly, so the oracle gets the same FP32-representable values)
+    from latticeboltzmann_amd import _native
+    from latticeboltzmann_amd.precision import DeviationLattice
+    w = o.weights()[:, None, None]
+    s = SCALES[fmt]
+    om, u0 = float(np.float32(1.3)), float(np.float32(0.1))
+    for shape in ((6, 7), (33, 300)):
+        f = populations(*shape)
+        for v in range(_native.load().lbm_dev_variant_count()):
+            lat = DeviationLattice(*shape, om, "cavity", u0, fmt=fmt, scale=s, variant=v,
+                                   rounding=rounding, compute="f64")
+            lat.load(f)
+            stored = lat.d.cpu().numpy().view(np.float32 if fmt == "f32" else np.uint16)
+            expect = lp.step_reference(stored, fmt, s, om, "cavity", u0, "rows")
+            lat.step(1)
+            x_exp, x_got = (expect - w) * s, (lat.populations() - w) * s
+            big = lp.ulp(np.abs(x_exp).max(0), fmt)
+            # FP32 storage is rounded with the compensation too under an FP64
+            # collide, so every format gets the compensated-rounding bounds
+            tol = (4 if rounding == "nearest" else 6) * big[None] + 1e-12
+            err = np.abs(x_got - x_exp)
+            assert (err <= tol).all(), (shape, v, float((err / tol).max()))
+
+
 @pytest.mark.parametrize("fmt", ["f16", "bf16"])
 def test_stochastic_rounding_unbiased_and_deterministic(gpu, fmt):
     # a uniform flow at equilibrium is a fixed point of the step; every node
