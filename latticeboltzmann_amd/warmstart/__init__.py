@@ -7,8 +7,9 @@ cavities the FP64 LBM itself produces (``data``, ``train``), predicts
 interpolates it to the N x N lattice and sets the populations
 (``solve.warm_populations``); the ``controller.Controller`` runs the LBM to
 steady state and falls back to a cold start when the warm start stalls,
-diverges or runs over budget.  The cold-started LBM stays the judge of the
-converged field (``evaluate``).  The reference has no counterpart.
+diverges or runs over budget.  The cold-started LBM stays the yardstick for the
+converged field (``evaluate``).  The reference repository has no learned
+counterpart.
 """
 
 from .controller import Controller

@@ -2,7 +2,7 @@
 
 For each cavity (Re, U drawn from a seed the training data did not use) the
 plain cold-started FP64 LBM runs to max|u(t+every) - u(t)|/U < tol and is the
-judge; the warm start (surrogate -> populations -> the same LBM, with the
+yardstick; the warm start (surrogate -> populations -> the same LBM, with the
 controller's cold-start fallback) runs to the same threshold.  Reported per
 cavity: steps and wall time of both (the warm time includes the network's
 inference and the initializer), whether it fell back, and the converged-field

@@ -14,7 +14,7 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free GPU guard: a gpu-marked test must run on a GPU box."""
+    """Skip-free GPU guard: a gpu-marked test must run where a GPU is visible."""
     import torch
     assert torch.cuda.is_available(), "gpu tests need a visible ROCm device"
     from latticeboltzmann_amd import _native

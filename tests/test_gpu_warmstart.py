@@ -1,6 +1,6 @@
 """The shipped, trained surrogate: what it gains over the cold start (and over
 an untrained network of the same shape) on held-out cavities, with the
-cold-started LBM as the judge of the converged field."""
+cold-started LBM as the yardstick for the converged field."""
 
 import numpy as np
 import pytest
