@@ -44,6 +44,11 @@ __global__ __launch_bounds__(BS) void k_batch_step(const typename Codec<FMT>::ra
         const int nx = I.nx, ny = I.ny;
         const int n = nx * ny;
         const Layout L{ny, 9 * int64_t(ny)};
+        // row of node j: (j + 1/2) * fl(1/ny) has relative error below 2.4e-7, so for
+        // n <= 2^21 it stays within 1/(2 ny) of the true quotient, whose fractional
+        // part lies in [1/(2 ny), 1 - 1/(2 ny)]: the truncation is exact
+        const bool fast_div = n <= (1 << 21);
+        const float inv_ny = 1.0f / float(ny);
         const R *s = src + I.offset;
         R *d = dst + I.offset;
 
@@ -55,7 +60,7 @@ __global__ __launch_bounds__(BS) void k_batch_step(const typename Codec<FMT>::ra
             kk[v] = -1;
             ll[v] = 0;
             if (j < n) {
-                const int k = j / ny;
+                const int k = fast_div ? int((float(j) + 0.5f) * inv_ny) : j / ny;
                 kk[v] = k;
                 ll[v] = j - k * ny;
                 const PullRows<R> rows(s, L, k, nx);
