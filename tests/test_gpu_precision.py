@@ -26,10 +26,10 @@ def populations(nx, ny, spread=0.05):
 
 
 def lattice(nx, ny, fmt, bc="cavity", omega=1.3, layout="rows", variant=-1,
-            rounding="stochastic", seed=0):
+            rounding="stochastic", seed=0, compute="f32"):
     from latticeboltzmann_amd.precision import DeviationLattice
     return DeviationLattice(nx, ny, omega, bc, 0.1, fmt=fmt, scale=SCALES[fmt], layout=layout,
-                            variant=variant, rounding=rounding, seed=seed)
+                            variant=variant, rounding=rounding, seed=seed, compute=compute)
 
 
 @pytest.mark.parametrize("fmt", lp.FORMATS)
@@ -115,17 +115,20 @@ def test_one_step_fp64_collide_against_oracle(gpu, fmt, rounding):
 
 
 @pytest.mark.parametrize("fmt", ["f16", "bf16"])
-def test_stochastic_rounding_unbiased_and_deterministic(gpu, fmt):
+@pytest.mark.parametrize("compute", ["f32", "f64"])
+def test_stochastic_rounding_unbiased_and_deterministic(gpu, fmt, compute):
     # a uniform flow at equilibrium is a fixed point of the step; every node
     # stores the same exact value, so nearest rounding has one fixed error per
-    # channel while stochastic rounding averages out over the nodes
+    # channel while stochastic rounding averages out over the nodes (with the
+    # FP64 collide the values are rounded stochastically straight from FP64)
     n = 256
     rho, ux, uy = 1.0, 0.05, -0.02
     feq = o.equilibrium(np.full((n, n), rho), np.full((n, n), ux), np.full((n, n), uy))
     s = SCALES[fmt]
     outs = {}
     for rounding, seed in (("nearest", 0), ("stochastic", 1), ("stochastic", 1), ("stochastic", 2)):
-        lat = lattice(n, n, fmt, bc="periodic", omega=1.0, rounding=rounding, seed=seed)
+        lat = lattice(n, n, fmt, bc="periodic", omega=1.0, rounding=rounding, seed=seed,
+                      compute=compute)
         lat.load(feq)
         # the exact result of the step is the FP64 step of the (rounded) stored input
         stored = lat.d.cpu().numpy().view(np.uint16)[:2, :, :2].copy()

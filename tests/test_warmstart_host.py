@@ -141,3 +141,19 @@ def test_dataset_density_is_mean_free(tmp_path):
 
 def test_layer_table_matches_oracle():
     assert unet_layers(6, (32, 64, 128), 128, 3) == s.unet_layers(6, (32, 64, 128), 128, 3)
+
+
+def test_evaluation_summary():
+    from latticeboltzmann_amd.warmstart.evaluate import summarize
+    recs = []
+    for i, (cs, ws, cst, wst, d, fb) in enumerate([(100, 40, 1.0, 0.40, 2e-5, False),
+                                                     (200, 50, 2.0, 0.55, 5e-5, False),
+                                                     (300, 150, 3.0, 1.60, 2e-4, True)]):
+        recs.append({"cold": {"steps": cs, "seconds": cst},
+                     "warm": {"steps": ws, "seconds": wst, "fell_back": fb, "diff_max": d,
+                              "step_speedup": cs / ws, "time_speedup": cst / wst}})
+    w = summarize(recs)["warm"]
+    assert w["median_step_speedup"] == 2.5 and w["fallbacks"] == 1
+    assert np.isclose(w["total_time_speedup"], 6.0 / 2.55)
+    assert np.isclose(w["min_time_speedup"], 3.0 / 1.6)
+    assert w["diff_max_worst"] == 2e-4 and np.isclose(w["frac_diff_below_1e-4"], 2 / 3)
