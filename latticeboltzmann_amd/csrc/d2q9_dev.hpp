@@ -63,15 +63,30 @@ __device__ __forceinline__ uint32_t hash32(uint32_t x) {
     return x;
 }
 
-// Per-node random stream: one hash of (seed, node), then an LCG per value
-// whose high 16 bits supply the dither.
+// Per-node random stream: one hash of (seed, node) seeds a xorshift32
+// generator (Marsaglia, 2003; shifts 13, 17, 5), whose states are used 16 bits
+// at a time.  Shifts and xors run at full rate, unlike the 32-bit multiply of a
+// linear congruential step (quarter rate), and the nine to eighteen dithers of
+// a node are the store path's largest cost.  The state is forced odd (a
+// xorshift state of 0 would stay 0).  Within one step (one seed) distinct
+// nodes get distinct streams.
 struct Dither {
-    uint32_t s;
+    uint32_t s, spare;
+    bool have;
     __device__ __forceinline__ Dither(uint32_t seed, uint64_t node)
-        : s(hash32(seed ^ hash32(uint32_t(node) ^ (uint32_t(node >> 32) * 0x9e3779b9U)))) {}
+        : s(hash32(seed ^ uint32_t(node) ^ (uint32_t(node >> 32) * 0x9e3779b9U)) | 1U), spare(0),
+          have(false) {}
     __device__ __forceinline__ uint32_t next16() {
-        s = s * 1664525U + 1013904223U;
-        return s >> 16;
+        if (have) {
+            have = false;
+            return spare;
+        }
+        s ^= s << 13;
+        s ^= s >> 17;
+        s ^= s << 5;
+        spare = s >> 16;
+        have = true;
+        return s & 0xffffU;
     }
 };
 
