@@ -227,6 +227,15 @@ def shear_wave_amplitude(f, direction="x"):
     return float(np.dot(line, np.sin(2 * np.pi * np.arange(nx) / nx)) * 2 / nx)
 
 
+def shear_wave_amplitude_fft(f, direction="x"):
+    """Amplitude as ``shear_wave_decay.py:155-158`` measures it: the largest
+    entry of (2/n) |FFT(u)| over the line through the middle of the box."""
+    _, ux, uy = moments(f.astype(np.float64))
+    nx, ny = ux.shape
+    line = ux[nx // 2, :] if direction == "x" else uy[:, ny // 2]
+    return float(np.max(2.0 / line.shape[0] * np.abs(np.fft.fft(line))))
+
+
 def viscosity_theory(omega):
     """nu = (1/omega - 1/2)/3 (``shear_wave_decay.py:162``)."""
     return (1.0 / omega - 0.5) / 3.0

@@ -282,3 +282,11 @@ def test_shear_wave_decay_matches_viscosity(direction):
     nu = o.fit_viscosity(np.arange(steps + 1), amps, n)
     assert abs(nu / o.viscosity_theory(omega) - 1) < 2e-3
     assert abs(amps[0] - 0.05) < 1e-12
+
+
+def test_fft_amplitude_of_a_pure_sine():
+    # (2/n) |FFT| of A sin(2 pi j / n) peaks at A (k = +-1), for any phase of the line
+    for direction, shape in (("x", (8, 64)), ("y", (48, 6))):
+        f = o.shear_wave_init(*shape, amplitude=0.07, direction=direction)
+        assert abs(o.shear_wave_amplitude_fft(f, direction) - 0.07) < 1e-15
+        assert abs(o.shear_wave_amplitude(f, direction) - 0.07) < 1e-15
