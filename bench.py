@@ -10,7 +10,8 @@ events on the stream the kernels are launched on.
 Also reported (``extra``): the same kernel on an 8192 x 8192 cavity (past the
 256 MB last-level cache: the size where an HBM-roofline fraction means
 something) in FP64 and with FP32/BF16/FP16 deviation storage; a
-device-to-device copy of the same byte count; the batched sweep of 512
+device-to-device copy of the same byte count (torch's, and the library's own
+16-byte copy, the practical ceiling of a bandwidth-bound kernel); the batched sweep of 512
 shear-wave instances (seed 0, FP32 storage, 2000 steps, one persistent
 graph-captured launch per step) with its fitted-viscosity errors; and the
 warm-start time to steady state against the cold start on held-out 256 x 256
@@ -92,6 +93,32 @@ def timed_steps(lat, steps, warmup):
     end.record()
     torch.cuda.synchronize()
     return start.elapsed_time(end) / 1e3  # seconds
+
+
+def copy_ceiling(nbytes, reps=20):
+    """Best HBM rate of lbm_copy (16-byte words, plain or non-temporal stores) on
+    nbytes, in GB/s counting bytes read plus written: the practical ceiling of a
+    bandwidth-bound kernel on this device."""
+    import torch
+    from latticeboltzmann_amd import _native
+    a = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    best = 0.0
+    for nt in (0, 1):
+        def run():
+            _native.call("lbm_copy", a.data_ptr(), b.data_ptr(), nbytes, nt, _native.stream_ptr())
+        run()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run()
+        e1.record()
+        torch.cuda.synchronize()
+        best = max(best, 2 * nbytes / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9)
+    del a, b
+    torch.cuda.empty_cache()
+    return best
 
 
 def copy_bandwidth(nbytes):
@@ -242,6 +269,9 @@ def main():
         extra["cavity_8192"] = legs
         nbytes = 9 * big * big * 8
         extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
+        ceiling = copy_ceiling(nbytes)
+        extra["copy_ceiling_gbs"] = round(ceiling, 1)
+        extra["f64_8192_over_copy_ceiling"] = round(legs["f64"]["achieved_gbs"] / ceiling, 3)
     if not args.no_batch:
         extra["batch_shear_512"] = batch_sweep()
     if not args.no_warm:

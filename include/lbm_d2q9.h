@@ -87,6 +87,12 @@ int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *r
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
                         int elem_bytes, void *stream);
 
+/* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with
+ * 16-byte loads and stores, non-temporal stores if nontemporal != 0.  No
+ * reference counterpart: the achievable HBM copy rate, against which the
+ * bandwidth-bound step kernels are measured.                                    */
+int lbm_copy(const void *src, void *dst, int64_t nbytes, int nontemporal, void *stream);
+
 /* Copy a population array between layouts (a permutation; src, dst disjoint).
  * No reference counterpart: the reference has one layout.                      */
 int lbm_relayout(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,

@@ -45,6 +45,7 @@ SIGNATURES = {
     "lbm_collide_f32": (_i32, [_vp, _i64, _f32, _vp, _vp, _vp, _vp]),
     "lbm_moments_f64": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
     "lbm_moments_f32": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
+    "lbm_copy": (_i32, [_vp, _vp, _i64, _i32, _vp]),
     "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),
     "lbm_stream_periodic": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp]),
     "lbm_stream_bounce_back_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp]),

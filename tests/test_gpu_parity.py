@@ -302,3 +302,18 @@ def test_graph_replayed_steady_run_equals_eager(gpu):
     sb, rb = b.run_until_steady(tol=1e-4, every=1000, graph=False)
     assert sa == sb and ra == rb and a.time == b.time == sa
     assert torch.equal(a.f, b.f)
+
+
+def test_copy_kernel(gpu):
+    from latticeboltzmann_amd import _native as nat
+    lib = nat.load()
+    a = torch.arange(10000, dtype=torch.float64, device="cuda")
+    for nt in (0, 1):
+        b = torch.zeros_like(a)
+        nat.call("lbm_copy", a.data_ptr(), b.data_ptr(), 8 * 9998, nt, nat.stream_ptr())
+        assert torch.equal(b[:9998], a[:9998]) and float(b[9998:].abs().sum()) == 0.0
+    b = torch.zeros_like(a)
+    s_ = nat.stream_ptr()
+    assert lib.lbm_copy(a.data_ptr(), b.data_ptr(), 8 * 9999, 0, s_) == -2        # not a multiple of 16
+    assert lib.lbm_copy(a.data_ptr() + 8, b.data_ptr(), 16, 0, s_) == -3          # misaligned
+    assert lib.lbm_copy(a.data_ptr(), a.data_ptr() + 16, 64, 0, s_) == -4         # overlap
