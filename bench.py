@@ -55,6 +55,7 @@ def parse():
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep leg")
     ap.add_argument("--no-warm", action="store_true", help="skip the warm-start leg")
+    ap.add_argument("--no-dumps", action="store_true", help="skip the velocity-dump leg")
     ap.add_argument("--warm-count", type=int, default=6)
     ap.add_argument("--cpu-budget", type=float, default=8.0)
     ap.add_argument("--copy-only", action="store_true",
@@ -194,6 +195,37 @@ def batch_sweep(steps=2000):
     return out
 
 
+def dumped_runs(n, omega, steps, plain_s, freqs=(10000, 1000)):
+    """The cavity run again with the reference's velocity dumps (ux_i.npy,
+    uy_i.npy every dump_freq steps and at the last step) written to a
+    temporary directory; wall time against the plain run of the same steps."""
+    import shutil
+    import tempfile
+    import torch
+    from latticeboltzmann_amd.dumps import run_cavity
+    from latticeboltzmann_amd.lattice import Lattice
+    out = {}
+    for freq in freqs:
+        lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant="auto")
+        lat.step(20)
+        torch.cuda.synchronize()
+        d = tempfile.mkdtemp(prefix="lbm_dumps_")
+        try:
+            t0 = time.perf_counter()
+            files = run_cavity(lat, steps, freq, d)
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+            nbytes = sum(os.path.getsize(f) for f in files)
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        out["every_%d" % freq] = {"files": len(files), "bytes": nbytes, "seconds": round(sec, 4),
+                                  "mlups": round(n * n * steps / sec / 1e6, 1),
+                                  "overhead_vs_plain": round(sec / plain_s - 1.0, 4)}
+        del lat
+        torch.cuda.empty_cache()
+    return out
+
+
 def warm_start(count, n=256, seed=2000):
     """Held-out cavities: cold vs warm-started steps and wall time to steady state."""
     from latticeboltzmann_amd.warmstart.controller import Controller
@@ -272,6 +304,8 @@ def main():
         ceiling = copy_ceiling(nbytes)
         extra["copy_ceiling_gbs"] = round(ceiling, 1)
         extra["f64_8192_over_copy_ceiling"] = round(legs["f64"]["achieved_gbs"] / ceiling, 3)
+    if not args.no_dumps:
+        extra["dumps_%d" % n] = dumped_runs(n, omega, args.steps, sec)
     if not args.no_batch:
         extra["batch_shear_512"] = batch_sweep()
     if not args.no_warm:
