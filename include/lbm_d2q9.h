@@ -330,6 +330,65 @@ int lbm_nn_to_bf16(const float *x, void *y, int64_t n, void *stream);
 int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, double omega,
                       int flags, int pool, double *f, void *stream);
 
+/* ---- ghost-layer flows --------------------------------------------------------------
+ * The reference's simple_flows / experimantal_flows scripts (Couette and
+ * Poiseuille channels, PoiseuilleFlow.py:59-150; sliding lid, slidingLid.py:
+ * 68-108; obstacle canal, obstacle_canal.py:265-458) keep a ghost layer inside
+ * the (9, nx, ny) array (layout f[i][k][l]) and per step:
+ *   1. optionally replace columns 0 and nx-1 by the pressure-periodic inlet /
+ *      outlet values, f = f_eq(rho_in|out, u(column nx-2|1)) + f(column) -
+ *      f_eq(column), before streaming (PoiseuilleFlow.py:76-89) or after it
+ *      ((f_eq + f) - f_eq, obstacle_canal.py:362-373);
+ *   2. stream periodically over the whole array (np.roll);
+ *   3. overwrite populations by the rules, in order: target nodes x0 <= x < x1,
+ *      y0 <= y < y1, channel ch, take the streamed value of channel src_ch at
+ *      (x + dx, y + dy), plus coef (LBM_RULE_LID) or coef times the wall
+ *      density 2 (g2 + g5 + g6) + g0 + g1 + g3 at the source node
+ *      (LBM_RULE_LID_RHO, slidingLid.py:88-91), or 0 (LBM_RULE_ZERO);
+ *   4. collide every node with the scripts' equilibrium arithmetic
+ *      (PoiseuilleFlow.py:25-53): f -= omega (f - f_eq).
+ * One launch reads each population once and writes it once (src -> dst).  A
+ * rule may not read a population an earlier rule writes (LBM_E_ARG), so every
+ * rule reads the streamed values.  rules and pbc are host pointers, copied
+ * into the launch.  flags: LBM_GHOST_NO_COLLIDE stops after step 3,
+ * LBM_GHOST_COLLIDE_ONLY does step 4 alone.  rho/ux/uy (nullable, all or
+ * none) receive the moments before the collision.  nx, ny >= 3, nx <= 65535.
+ * Replaces the step functions of those scripts (no PyLB counterpart).         */
+#define LBM_RULE_COPY 0
+#define LBM_RULE_LID 1
+#define LBM_RULE_LID_RHO 2
+#define LBM_RULE_ZERO 3
+#define LBM_GHOST_MAX_RULES 32
+#define LBM_PBC_NONE 0
+#define LBM_PBC_BEFORE_STREAM 1
+#define LBM_PBC_AFTER_STREAM 2
+#define LBM_GHOST_NO_COLLIDE 1
+#define LBM_GHOST_COLLIDE_ONLY 2
+typedef struct lbm_ghost_rule {
+    int32_t x0, x1, y0, y1; /* target nodes                                    */
+    int32_t ch, kind;       /* target channel, LBM_RULE_*                      */
+    int32_t dx, dy, src_ch; /* source node offset and channel                  */
+    int32_t pad;
+    double coef; /* signed: -uw/6 or +uw/6 in the scripts                      */
+} lbm_ghost_rule;
+typedef struct lbm_ghost_pbc {
+    int32_t mode; /* LBM_PBC_*                                                 */
+    int32_t pad;
+    double rho_in, rho_out;
+} lbm_ghost_pbc;
+/* f[i][k] = the scripts' equilibrium of (rho, ux, uy)[k], k < n
+ * (PoiseuilleFlow.py:25-43, the initial state of every ghost-layer flow).       */
+int lbm_ghost_equilibrium_f64(const double *rho, const double *ux, const double *uy, double *f,
+                              int64_t n, void *stream);
+int lbm_ghost_equilibrium_f32(const float *rho, const float *ux, const float *uy, float *f,
+                              int64_t n, void *stream);
+int lbm_ghost_step_f64(const double *src, double *dst, int64_t nx, int64_t ny, double omega,
+                       const lbm_ghost_rule *rules, int nrules, const lbm_ghost_pbc *pbc,
+                       int flags, double *rho, double *ux, double *uy, void *stream);
+int lbm_ghost_step_f32(const float *src, float *dst, int64_t nx, int64_t ny, double omega,
+                       const lbm_ghost_rule *rules, int nrules, const lbm_ghost_pbc *pbc,
+                       int flags, float *rho, float *ux, float *uy, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,6 +35,26 @@ WARM_NEQ, WARM_WALLS, WARM_CUBIC = 1, 2, 4
 _vp, _i64, _i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int
 _f64, _f32 = ctypes.c_double, ctypes.c_float
 
+RULE_COPY, RULE_LID, RULE_LID_RHO, RULE_ZERO = 0, 1, 2, 3
+GHOST_MAX_RULES = 32
+PBC_NONE, PBC_BEFORE_STREAM, PBC_AFTER_STREAM = 0, 1, 2
+GHOST_NO_COLLIDE, GHOST_COLLIDE_ONLY = 1, 2
+
+
+class GhostRule(ctypes.Structure):
+    """lbm_ghost_rule (include/lbm_d2q9.h)."""
+    _fields_ = [("x0", ctypes.c_int32), ("x1", ctypes.c_int32), ("y0", ctypes.c_int32),
+                ("y1", ctypes.c_int32), ("ch", ctypes.c_int32), ("kind", ctypes.c_int32),
+                ("dx", ctypes.c_int32), ("dy", ctypes.c_int32), ("src_ch", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("coef", ctypes.c_double)]
+
+
+class GhostPbc(ctypes.Structure):
+    """lbm_ghost_pbc (include/lbm_d2q9.h)."""
+    _fields_ = [("mode", ctypes.c_int32), ("pad", ctypes.c_int32), ("rho_in", ctypes.c_double),
+                ("rho_out", ctypes.c_double)]
+
+
 # name -> (restype, argtypes); keep in step with include/lbm_d2q9.h
 SIGNATURES = {
     "lbm_version": (ctypes.c_char_p, []),
@@ -104,6 +124,12 @@ SIGNATURES = {
     "lbm_nn_ema": (_i32, [_vp, _vp, _i64, _f32, _vp]),
     "lbm_nn_to_bf16": (_i32, [_vp, _vp, _i64, _vp]),
     "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _i32, _vp, _vp]),
+    "lbm_ghost_equilibrium_f64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "lbm_ghost_equilibrium_f32": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
+    "lbm_ghost_step_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp, _i32, _vp, _i32, _vp, _vp,
+                                  _vp, _vp]),
+    "lbm_ghost_step_f32": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp, _i32, _vp, _i32, _vp, _vp,
+                                  _vp, _vp]),
 }
 
 _lib = None

@@ -3,8 +3,8 @@
 Each ``csrc/*.hip`` translation unit is compiled for gfx950 with its own
 flags and the objects are linked into one shared library next to this file.
 
-* ``d2q9_kernels.hip`` (the bit-exact FP64/FP32 path) is compiled with
-  ``-ffp-contract=off``: numpy never fuses a multiply into an add, so neither
+* ``d2q9_kernels.hip`` and ``ghost.hip`` (the bit-exact FP64/FP32 paths) are
+  compiled with ``-ffp-contract=off``: numpy never fuses a multiply into an add, so neither
   may the kernels that must reproduce it.
 * every unit is compiled with the ``packed-fp32-ops`` target feature switched
   off, so the compiler never emits packed FP32 arithmetic.
@@ -39,6 +39,7 @@ UNITS = {
     "d2q9_lowprec.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
     "d2q9_batch.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
     "surrogate.hip": ["-ffp-contract=fast"],
+    "ghost.hip": ["-ffp-contract=off"],
 }
 
 
