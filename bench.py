@@ -226,6 +226,24 @@ def dumped_runs(n, omega, steps, plain_s, freqs=(10000, 1000)):
     return out
 
 
+def ghost_flow(n=4096, steps=200):
+    """The sliding-lid script's flow (ghost-layer walls, slidingLid.py) on
+    (n + 2)^2 nodes in FP64: one fused ghost-layer step per launch."""
+    import torch
+    from latticeboltzmann_amd import ghost
+    lat = ghost.GhostLattice(n + 2, n + 2, ghost.sliding_lid_omega(n, 1000, 0.1),
+                             ghost.cavity_walls(n + 2, n + 2, 0.1))
+    s = timed_steps(lat, steps, 20)
+    nodes = (n + 2) ** 2
+    m = nodes * steps / s / 1e6
+    gbs = BYTES_PER_LUP["f64"] * m * 1e6 / 1e9
+    ok = bool(torch.isfinite(lat.f).all())
+    del lat
+    torch.cuda.empty_cache()
+    return {"nodes": nodes, "steps": steps, "mlups": round(m, 1), "achieved_gbs": round(gbs, 1),
+            "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4), "finite": ok}
+
+
 def warm_start(count, n=256, seed=2000):
     """Held-out cavities: cold vs warm-started steps and wall time to steady state."""
     from latticeboltzmann_amd.warmstart.controller import Controller
@@ -304,6 +322,7 @@ def main():
         ceiling = copy_ceiling(nbytes)
         extra["copy_ceiling_gbs"] = round(ceiling, 1)
         extra["f64_8192_over_copy_ceiling"] = round(legs["f64"]["achieved_gbs"] / ceiling, 3)
+        extra["ghost_sliding_lid_4098"] = ghost_flow()
     if not args.no_dumps:
         extra["dumps_%d" % n] = dumped_runs(n, omega, args.steps, sec)
     if not args.no_batch:

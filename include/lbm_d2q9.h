@@ -334,7 +334,7 @@ int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, d
  * The reference's simple_flows / experimantal_flows scripts (Couette and
  * Poiseuille channels, PoiseuilleFlow.py:59-150; sliding lid, slidingLid.py:
  * 68-108; obstacle canal, obstacle_canal.py:265-458) keep a ghost layer inside
- * the (9, nx, ny) array (layout f[i][k][l]) and per step:
+ * the (9, nx, ny) array and per step:
  *   1. optionally replace columns 0 and nx-1 by the pressure-periodic inlet /
  *      outlet values, f = f_eq(rho_in|out, u(column nx-2|1)) + f(column) -
  *      f_eq(column), before streaming (PoiseuilleFlow.py:76-89) or after it
@@ -347,7 +347,9 @@ int lbm_warm_init_f64(const float *q, int m, int64_t nx, int64_t ny, double U, d
  *      (LBM_RULE_LID_RHO, slidingLid.py:88-91), or 0 (LBM_RULE_ZERO);
  *   4. collide every node with the scripts' equilibrium arithmetic
  *      (PoiseuilleFlow.py:25-53): f -= omega (f - f_eq).
- * One launch reads each population once and writes it once (src -> dst).  A
+ * One launch reads each population once and writes it once (src -> dst), in
+ * either layout (LBM_LAYOUT_SOA is the scripts' own; LBM_LAYOUT_ROWS streams
+ * faster; the results are the same bits); rho/ux/uy are [k][l].  A
  * rule may not read a population an earlier rule writes (LBM_E_ARG), so every
  * rule reads the streamed values.  rules and pbc are host pointers, copied
  * into the launch.  flags: LBM_GHOST_NO_COLLIDE stops after step 3,
@@ -382,12 +384,14 @@ int lbm_ghost_equilibrium_f64(const double *rho, const double *ux, const double 
                               int64_t n, void *stream);
 int lbm_ghost_equilibrium_f32(const float *rho, const float *ux, const float *uy, float *f,
                               int64_t n, void *stream);
-int lbm_ghost_step_f64(const double *src, double *dst, int64_t nx, int64_t ny, double omega,
-                       const lbm_ghost_rule *rules, int nrules, const lbm_ghost_pbc *pbc,
-                       int flags, double *rho, double *ux, double *uy, void *stream);
-int lbm_ghost_step_f32(const float *src, float *dst, int64_t nx, int64_t ny, double omega,
-                       const lbm_ghost_rule *rules, int nrules, const lbm_ghost_pbc *pbc,
-                       int flags, float *rho, float *ux, float *uy, void *stream);
+int lbm_ghost_step_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout,
+                       double omega, const lbm_ghost_rule *rules, int nrules,
+                       const lbm_ghost_pbc *pbc, int flags, double *rho, double *ux, double *uy,
+                       void *stream);
+int lbm_ghost_step_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout,
+                       double omega, const lbm_ghost_rule *rules, int nrules,
+                       const lbm_ghost_pbc *pbc, int flags, float *rho, float *ux, float *uy,
+                       void *stream);
 
 #ifdef __cplusplus
 }

@@ -126,10 +126,10 @@ SIGNATURES = {
     "lbm_warm_init_f64": (_i32, [_vp, _i32, _i64, _i64, _f64, _f64, _i32, _i32, _vp, _vp]),
     "lbm_ghost_equilibrium_f64": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
     "lbm_ghost_equilibrium_f32": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp]),
-    "lbm_ghost_step_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp, _i32, _vp, _i32, _vp, _vp,
-                                  _vp, _vp]),
-    "lbm_ghost_step_f32": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp, _i32, _vp, _i32, _vp, _vp,
-                                  _vp, _vp]),
+    "lbm_ghost_step_f64": (_i32, [_vp, _vp, _i64, _i64, _i32, _f64, _vp, _i32, _vp, _i32, _vp,
+                                  _vp, _vp, _vp]),
+    "lbm_ghost_step_f32": (_i32, [_vp, _vp, _i64, _i64, _i32, _f64, _vp, _i32, _vp, _i32, _vp,
+                                  _vp, _vp, _vp]),
 }
 
 _lib = None

@@ -264,9 +264,9 @@ __global__ __launch_bounds__(kBlock) void k_residual_pass1(const double *__restr
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// Plain copy of n16 16-byte words, four per thread (strided by the block), with
-// ordinary or non-temporal stores: the achievable HBM copy rate on this device,
-// the practical ceiling the step kernels are compared against.
+// Plain copy of n16 16-byte words with ordinary or non-temporal stores: the
+// achievable HBM copy rate on this device, the practical ceiling the step
+// kernels are compared against.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool NT>

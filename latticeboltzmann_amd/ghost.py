@@ -34,6 +34,7 @@ import numpy as np
 import torch
 
 from . import _native as nat
+from . import kernels
 
 _SUFFIX = {torch.float64: "f64", torch.float32: "f32"}
 
@@ -150,16 +151,21 @@ def pulse_walls(nx, ny, vertical):
 
 
 class GhostLattice:
-    """A ``(9, nx, ny)`` ghost-layer lattice on one GPU (layout f[i][k][l]).
+    """A ghost-layer lattice of nx x ny nodes on one GPU.
 
     ``rules``: override rules in order (at most 32); ``pbc``: None, or
     ``("before" | "after", rho_in, rho_out)`` for the pressure-periodic inlet
     (column 0) and outlet (column nx-1), built before streaming
     (``PoiseuilleFlow.py:76-89``) or after it (``obstacle_canal.py:362-373``).
     The state starts at rest: the scripts' equilibrium of rho = 1, u = 0.
+    ``layout``: ``"rows"`` (default; f[k][i][l], the nine channel segments of
+    an x-row adjacent, faster) or ``"soa"`` (the scripts' f[i][k][l]); the
+    results are the same bits.  ``populations()`` and ``load()`` use the
+    scripts' ``(9, nx, ny)`` either way.
     """
 
-    def __init__(self, nx, ny, omega, rules=(), pbc=None, dtype=torch.float64, device=None):
+    def __init__(self, nx, ny, omega, rules=(), pbc=None, dtype=torch.float64, device=None,
+                 layout="rows"):
         nat.require_gpu()
         if dtype not in _SUFFIX:
             raise ValueError("dtype must be torch.float64 or torch.float32")
@@ -180,7 +186,11 @@ class GhostLattice:
             if mode not in modes:
                 raise ValueError("pbc mode must be 'before' or 'after'")
             self._pbc = nat.GhostPbc(modes[mode], 0, float(rho_in), float(rho_out))
-        self.f = torch.empty((9, self.nx, self.ny), dtype=dtype, device=self.device)
+        if layout not in ("rows", "soa"):
+            raise ValueError("layout must be 'rows' or 'soa'")
+        self.layout = nat.LAYOUT_ROWS if layout == "rows" else nat.LAYOUT_SOA
+        shape = (self.nx, 9, self.ny) if self.layout == nat.LAYOUT_ROWS else (9, self.nx, self.ny)
+        self.f = torch.empty(shape, dtype=dtype, device=self.device)
         self._g = torch.empty_like(self.f)
         self.time = 0
         self.init_equilibrium()
@@ -195,16 +205,25 @@ class GhostLattice:
             return torch.as_tensor(x, dtype=self.dtype, device=self.device).expand(shp).contiguous()
 
         r, u, v = field(rho, 1.0), field(ux, 0.0), field(uy, 0.0)
+        soa = torch.empty((9, self.nx, self.ny), dtype=self.dtype, device=self.device)
         nat.call("lbm_ghost_equilibrium_" + _SUFFIX[self.dtype], nat.ptr(r), nat.ptr(u),
-                 nat.ptr(v), nat.ptr(self.f), self.nx * self.ny, nat.stream_ptr())
+                 nat.ptr(v), nat.ptr(soa), self.nx * self.ny, nat.stream_ptr())
+        self._from_soa(soa)
         self.time = 0
         return self
 
+    def _from_soa(self, soa):
+        if self.layout == nat.LAYOUT_SOA:
+            self.f.copy_(soa)
+        else:
+            kernels.relayout(soa, nat.LAYOUT_SOA, nat.LAYOUT_ROWS, out=self.f)
+
     def load(self, f):
+        """Replace the populations by ``f`` (shape (9, nx, ny))."""
         t = torch.as_tensor(np.asarray(f) if not isinstance(f, torch.Tensor) else f)
         if tuple(t.shape) != (9, self.nx, self.ny):
             raise ValueError("populations must have shape (9, %d, %d)" % (self.nx, self.ny))
-        self.f.copy_(t.to(device=self.device, dtype=self.dtype))
+        self._from_soa(t.to(device=self.device, dtype=self.dtype).contiguous())
         return self
 
     def _launch(self, flags, moments):
@@ -212,7 +231,8 @@ class GhostLattice:
         if moments is not None:
             rho, ux, uy = moments
         nat.call("lbm_ghost_step_" + _SUFFIX[self.dtype], nat.ptr(self.f), nat.ptr(self._g),
-                 self.nx, self.ny, self.omega, ctypes.addressof(self._rules), len(self.rules),
+                 self.nx, self.ny, self.layout, self.omega, ctypes.addressof(self._rules),
+                 len(self.rules),
                  ctypes.addressof(self._pbc) if self._pbc is not None else None, int(flags),
                  nat.ptr(rho), nat.ptr(ux), nat.ptr(uy), nat.stream_ptr())
         self.f, self._g = self._g, self.f
@@ -238,17 +258,24 @@ class GhostLattice:
         self._launch(nat.GHOST_NO_COLLIDE, None)
         return self
 
+    def populations_device(self):
+        """Populations (9, nx, ny) as a device tensor."""
+        if self.layout == nat.LAYOUT_SOA:
+            return self.f.clone()
+        return kernels.relayout(self.f, nat.LAYOUT_ROWS, nat.LAYOUT_SOA)
+
     def populations(self):
-        return self.f.cpu().numpy()
+        """Populations (9, nx, ny) as a numpy array."""
+        return self.populations_device().cpu().numpy()
 
 
 # ---- the scripts' flows ----------------------------------------------------------
 
-def couette(nx=100, ny=52, uw=0.1, omega=0.5, steps=4000, dtype=torch.float64):
+def couette(nx=100, ny=52, uw=0.1, omega=0.5, steps=4000, dtype=torch.float64, layout="rows"):
     """``PoiseuilleFlow.py:93-113``: per step moments, collide, stream, walls
     (the state after the walls is the loop's state).  Returns the lattice and
     the moments of the last collision (what the script plots)."""
-    lat = GhostLattice(nx, ny, omega, channel_walls(nx, ny, uw), dtype=dtype)
+    lat = GhostLattice(nx, ny, omega, channel_walls(nx, ny, uw), dtype=dtype, layout=layout)
     mom = lat.moment_buffers()
     if steps <= 0:
         return lat, None
@@ -261,11 +288,11 @@ def couette(nx=100, ny=52, uw=0.1, omega=0.5, steps=4000, dtype=torch.float64):
 
 
 def poiseuille(nx=102, ny=52, rho_in=1.001, rho_out=0.999, omega=0.5, steps=4000, uw=0.0,
-               dtype=torch.float64):
+               dtype=torch.float64, layout="rows"):
     """``PoiseuilleFlow.py:129-150``: pressure-periodic columns before
     streaming, channel walls, collide."""
     lat = GhostLattice(nx, ny, omega, channel_walls(nx, ny, uw), ("before", rho_in, rho_out),
-                       dtype=dtype)
+                       dtype=dtype, layout=layout)
     mom = lat.moment_buffers()
     lat.step(steps, moments=mom)
     return lat, mom
@@ -276,18 +303,20 @@ def sliding_lid_omega(n, re, uw):
     return (2 * re) / (6 * n * uw + re)
 
 
-def sliding_lid(n=300, re=1000, uw=0.1, steps=10000, dtype=torch.float64, omega=None):
+def sliding_lid(n=300, re=1000, uw=0.1, steps=10000, dtype=torch.float64, omega=None,
+                layout="rows"):
     """``slidingLid.py:94-108``: (n + 2)^2 nodes, walls in the ghost layers, lid
     at the top moving with uw."""
     om = sliding_lid_omega(n, re, uw) if omega is None else omega
-    lat = GhostLattice(n + 2, n + 2, om, cavity_walls(n + 2, n + 2, uw), dtype=dtype)
+    lat = GhostLattice(n + 2, n + 2, om, cavity_walls(n + 2, n + 2, uw), dtype=dtype,
+                       layout=layout)
     mom = lat.moment_buffers()
     lat.step(steps, moments=mom)
     return lat, mom
 
 
 def obstacle_canal(nx=100, ny=50, obstacle=(50, 24, 53, 27), rho_diff=0.01, omega=0.5, steps=1000,
-                   uw=0.0, dtype=torch.float64):
+                   uw=0.0, dtype=torch.float64, layout="rows"):
     """``obstacle_canal.py:265-276`` on one process: (nx + 2) x (ny + 2)
     nodes, pressure-periodic left / right after streaming, walls at the top and
     bottom, an obstacle (sx, sy, ex, ey) with bouncing faces."""
@@ -296,7 +325,8 @@ def obstacle_canal(nx=100, ny=50, obstacle=(50, 24, 53, 27), rho_diff=0.01, omeg
     if not (2 <= sx < ex and ex + 1 < gx - 1 and 3 <= sy < ey and ey + 1 < gy - 2):
         raise ValueError("the obstacle must lie inside the canal, clear of the walls")
     rules = side_walls(gx, gy, uw) + obstacle_faces(sx, sy, ex, ey)
-    lat = GhostLattice(gx, gy, omega, rules, ("after", 1 + rho_diff, 1 - rho_diff), dtype=dtype)
+    lat = GhostLattice(gx, gy, omega, rules, ("after", 1 + rho_diff, 1 - rho_diff), dtype=dtype,
+                       layout=layout)
     mom = lat.moment_buffers()
     lat.step(steps, moments=mom)
     return lat, mom

@@ -20,10 +20,11 @@ def same(lat, mom, G, omom):
 
 @pytest.mark.parametrize("tdt,ndt", DT)
 @pytest.mark.parametrize("steps", [1, 2, 37])
-def test_couette_matches_oracle(gpu, tdt, ndt, steps):
+@pytest.mark.parametrize("layout", ["rows", "soa"])
+def test_couette_matches_oracle(gpu, tdt, ndt, steps, layout):
     from latticeboltzmann_amd import ghost
-    print("couette", tdt, steps, flush=True)
-    lat, mom = ghost.couette(20, 12, 0.1, 1.3, steps, dtype=tdt)
+    print("couette", tdt, steps, layout, flush=True)
+    lat, mom = ghost.couette(20, 12, 0.1, 1.3, steps, dtype=tdt, layout=layout)
     G, omom = g.couette(20, 12, 0.1, 1.3, steps, dtype=ndt)
     same(lat, mom, G, omom)
 
@@ -39,20 +40,23 @@ def test_poiseuille_matches_oracle(gpu, tdt, ndt, steps):
 
 
 @pytest.mark.parametrize("tdt,ndt", DT)
-def test_sliding_lid_matches_oracle(gpu, tdt, ndt):
+@pytest.mark.parametrize("layout", ["rows", "soa"])
+def test_sliding_lid_matches_oracle(gpu, tdt, ndt, layout):
     from latticeboltzmann_amd import ghost
-    print("sliding lid", tdt, flush=True)
-    lat, mom = ghost.sliding_lid(16, steps=37, uw=0.1, omega=1.7, dtype=tdt)
+    print("sliding lid", tdt, layout, flush=True)
+    lat, mom = ghost.sliding_lid(16, steps=37, uw=0.1, omega=1.7, dtype=tdt, layout=layout)
     G, omom = g.sliding_lid(18, 18, 0.1, 1.7, 37, dtype=ndt)
     same(lat, mom, G, omom)
 
 
 @pytest.mark.parametrize("tdt,ndt", DT)
-def test_obstacle_canal_matches_oracle(gpu, tdt, ndt):
+@pytest.mark.parametrize("layout", ["rows", "soa"])
+def test_obstacle_canal_matches_oracle(gpu, tdt, ndt, layout):
     from latticeboltzmann_amd import ghost
-    print("obstacle canal", tdt, flush=True)
+    print("obstacle canal", tdt, layout, flush=True)
     ob = (12, 8, 15, 11)
-    lat, mom = ghost.obstacle_canal(30, 20, ob, rho_diff=0.01, omega=0.8, steps=37, dtype=tdt)
+    lat, mom = ghost.obstacle_canal(30, 20, ob, rho_diff=0.01, omega=0.8, steps=37, dtype=tdt,
+                                    layout=layout)
     G, omom = g.obstacle_canal(32, 22, 1 + 0.01, 1 - 0.01, 0.8, 37, ob, dtype=ndt)
     same(lat, mom, G, omom)
 
