@@ -87,6 +87,22 @@ int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *r
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
                         int elem_bytes, void *stream);
 
+/* Two full time steps src -> dst in one launch (temporal blocking): each
+ * block computes the first step on its tile and a one-node halo into LDS and
+ * the second from there, so HBM sees about one read and one write of each
+ * population per two steps.  Bitwise equal to two lbm_step_layout_* calls
+ * (same arithmetic, cavity_opt1.py:341-343 twice); rho/ux/uy (nullable, all
+ * or none) receive the pre-collision moments of the second step.  variant
+ * picks a tile shape (0 .. lbm_step2_variant_count()-1, -1 default); the grid
+ * must be at least lbm_step2_min_size(variant) (LBM_E_SIZE otherwise).        */
+int lbm_step2_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout, double omega,
+                  int bc, double u0, double *rho, double *ux, double *uy, int variant,
+                  void *stream);
+int lbm_step2_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout, float omega,
+                  int bc, float u0, float *rho, float *ux, float *uy, int variant, void *stream);
+int lbm_step2_variant_count(void);
+int lbm_step2_min_size(int variant, int64_t *min_nx, int64_t *min_ny);
+
 /* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with
  * 16-byte loads and stores, non-temporal stores if nontemporal != 0.  No
  * reference counterpart: the achievable HBM copy rate, against which the

@@ -172,6 +172,39 @@ def step(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
     return dst
 
 
+def step2_min_size(variant=-1):
+    """Smallest (nx, ny) the two-step kernel takes with tile shape ``variant``."""
+    import ctypes
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    nat.call("lbm_step2_min_size", int(variant), ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def step2(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
+          layout=nat.LAYOUT_SOA):
+    """Two fused steps ``src -> dst`` in one launch (temporal blocking),
+    bitwise equal to two ``step`` calls; ``moments`` receive the
+    pre-collision moments of the second step."""
+    nx, ny = _populations(src, "step2", layout)
+    _require_device(dst)
+    suf = _suffix(src, "step2")
+    _same(src, dst, "step2")
+    if dst.shape != src.shape:
+        raise ValueError("step2: output must match the input")
+    rho = ux = uy = None
+    if moments is not None:
+        rho, ux, uy = moments
+        _require_device(rho, ux, uy)
+        for t in (rho, ux, uy):
+            _same(src, t, "step2")
+            if t.shape != (nx, ny):
+                raise ValueError("step2: moment arrays must have shape (nx, ny)")
+    nat.call("lbm_step2_" + suf, nat.ptr(src), nat.ptr(dst), nx, ny, int(layout), float(omega),
+             int(bc), float(u0), nat.ptr(rho), nat.ptr(ux), nat.ptr(uy), int(variant),
+             nat.stream_ptr())
+    return dst
+
+
 class Workspace:
     """Device scratch of the two-pass reductions plus a device result slot."""
 
