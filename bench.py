@@ -1,27 +1,33 @@
-"""Benchmark of the fused D2Q9 stream-collide kernel on one MI355X.
+"""Benchmark of the fused D2Q9 stream-collide kernels on one MI355X.
 
 Workload (the line's ``value``): the 2048 x 2048 lid-driven cavity in FP64,
 Re = 1000, U = 0.1 (omega = 1/(3 U N / Re + 1/2) = 0.897343), started from
-rho = 1, u = 0 at f_eq.  A step is one fused pull stream + walls + collide
-pass over the whole grid (one kernel launch).  Populations stay in HBM; the
-timed region is K steps bracketed by device synchronisation, timed with HIP
-events on the stream the kernels are launched on.
+rho = 1, u = 0 at f_eq, K steps.  A step is one pull stream + walls + collide
+pass over the whole grid.  Runs of 2 to 4 steps go through one launch of the
+multi-step kernel (temporal blocking: the earlier steps of a tile and its
+halo stay in LDS), which computes the bits of as many single-step launches.  Populations stay
+in HBM; the timed region is K steps bracketed by device synchronisation,
+timed with HIP events on the stream the kernels are launched on.
 
-Also reported (``extra``): the same kernel on an 8192 x 8192 cavity (past the
-256 MB last-level cache: the size where an HBM-roofline fraction means
-something) in FP64 and with FP32/BF16/FP16 deviation storage; a
-device-to-device copy of the same byte count (torch's, and the library's own
-16-byte copy, the practical ceiling of a bandwidth-bound kernel); the batched sweep of 512
-shear-wave instances (seed 0, FP32 storage, 2000 steps, one persistent
-graph-captured launch per step) with its fitted-viscosity errors; and the
-warm-start time to steady state against the cold start on held-out 256 x 256
-cavities (when trained surrogate weights are present).
+Also reported (``extra``): the single-step kernel on the same workload; an
+8192 x 8192 cavity (past the 256 MB last-level cache) in FP64 and with
+FP32/BF16/FP16 deviation storage; a device-to-device copy of the 8192^2 FP64
+byte count (torch's, and the library's own 16-byte copy, the practical
+ceiling of a single-pass bandwidth-bound kernel); the ghost-layer sliding-lid
+flow at 4098^2; the run again with the reference's velocity dumps; the
+batched sweep of 512 shear-wave instances (seed 0, FP32 storage, 2000 steps,
+one persistent graph-captured launch per step) with its fitted-viscosity
+errors; and the warm-start time to steady state against the cold start on
+held-out 256 x 256 cavities.
 
 ``roofline``: algorithmic bytes per lattice update = 2 * 9 * 8 (read and
-write all nine FP64 populations once) = 144 B; achieved = 144 * nodes /
-average launch duration; peak = 8 TB/s (MI355X HBM3E).  ``traffic`` is the
-HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters when a
-counter summary is committed under profiles/ (null otherwise).
+write all nine FP64 populations once) = 144 B; achieved = 144 B * lattice
+updates per launch (K per node for a K-step launch) / average launch
+duration; peak = 8 TB/s (MI355X HBM3E).  A K-step launch moves about 1/K of
+its algorithmic bytes through HBM, so ``frac`` can exceed 1: ``traffic`` is
+the HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters (a
+committed summary under profiles/, null if absent), and
+``extra.hbm_gbs_from_counters`` the HBM rate it implies.
 
 ``cpu_baseline``: the oracle's numpy opt1 restatement of the reference path
 timed on one host core on a bounded sample of the same 2048^2 cavity.
@@ -49,8 +55,15 @@ def parse():
                     "20000 steps of the 2048^2 cavity workload)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--variant", default="auto", help="launch shape of the step kernel "
-                    "(an integer, or 'auto' for the autotuner)")
+    ap.add_argument("--variant", default="auto", help="launch shape of the single-step kernel "
+                    "(an integer, or 'auto' for the autotuner, which then also picks the "
+                    "multi-step kernel's shape)")
+    ap.add_argument("--multi-variant", default="default",
+                    help="shape (steps per launch, tile) of the multi-step kernel: an integer, "
+                         "'default' (the library's default shape, the one the committed "
+                         "counter summary covers) or 'auto' (autotuned)")
+    ap.add_argument("--single-step", action="store_true",
+                    help="time the single-step kernel only (no multi-step launches)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-extra", action="store_true", help="skip the 8192^2 and copy legs")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched sweep leg")
@@ -142,14 +155,18 @@ def copy_bandwidth(nbytes):
     return 2 * nbytes / sec / 1e9
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch from a committed counter summary, or None."""
+def profiled_traffic(name, variant=None):
+    """HBM bytes per launch from a committed counter summary (per variant when
+    the summary holds several), or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
-            return json.load(fh).get("hbm_bytes_per_launch")
+            j = json.load(fh)
     except (OSError, ValueError):
         return None
+    if variant is not None and "per_variant" in j:
+        return j["per_variant"].get(str(variant), {}).get("hbm_bytes_per_launch")
+    return j.get("hbm_bytes_per_launch")
 
 
 def cpu_baseline(n, budget):
@@ -286,17 +303,43 @@ def main():
     omega = omega_for_cavity(n, 1000.0, 0.1)
     if args.variant != "auto":
         args.variant = int(args.variant)
-    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant)
-    lat_variant = lat.variant
+    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant,
+                  multi_step=not args.single_step)
+    if not args.single_step and args.multi_variant != "auto":
+        lat.set_multi_variant(-1 if args.multi_variant == "default" else int(args.multi_variant))
+    lat_variant, multi_var, multi_k = lat.variant, lat.multi_variant, lat.multi_k
+    launches = lat.launches(args.steps)
     sec = timed_steps(lat, args.steps, args.warmup)
     mlups = n * n * args.steps / sec / 1e6
-    launch_s = sec / args.steps
-    achieved = BYTES_PER_LUP["f64"] * n * n / launch_s / 1e9
+    launch_s = sec / launches
+    # algorithmic bytes of one launch: 144 B per lattice update, K updates per
+    # node for a K-step launch
+    alg_per_launch = BYTES_PER_LUP["f64"] * n * n * args.steps / launches
+    achieved = alg_per_launch / launch_s / 1e9
     ok = bool(torch.isfinite(lat.f).all())
     del lat
     torch.cuda.empty_cache()
+    if multi_var is not None:
+        dominant = "k_stepk (%d steps per launch, variant %d)" % (multi_k, multi_var)
+        traffic = profiled_traffic("step_multi_f64_cavity_%d_traffic.json" % n, multi_var)
+    else:
+        dominant = "k_step"
+        traffic = profiled_traffic("step_f64_cavity_%d_traffic.json" % n)
 
-    extra = {"finite_state": ok, "omega": omega, "variant": lat_variant}
+    extra = {"finite_state": ok, "omega": omega, "variant": lat_variant,
+             "multi_variant": multi_var, "steps_per_launch": multi_k, "dominant_kernel": dominant,
+             "launches": launches, "algorithmic_bytes_per_launch": int(alg_per_launch)}
+    if traffic:
+        extra["hbm_gbs_from_counters"] = round(traffic / launch_s / 1e9, 1)
+        extra["hbm_frac_from_counters"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+    if multi_var is not None and not args.no_extra:
+        lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant,
+                      multi_step=False)
+        s1 = timed_steps(lat, 2000, 50)
+        del lat
+        torch.cuda.empty_cache()
+        extra["single_step_%d" % n] = {"mlups": round(n * n * 2000 / s1 / 1e6, 1),
+                                       "achieved_gbs": round(144 * n * n * 2000 / s1 / 1e9, 1)}
     if not args.no_extra:
         from latticeboltzmann_amd.precision import DeviationLattice
         big = 8192
@@ -304,7 +347,8 @@ def main():
         legs = {}
         for name in ("f64", "f32", "bf16", "f16"):
             if name == "f64":
-                lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant)
+                lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant,
+                              multi_step=not args.single_step)
             else:
                 lat = DeviationLattice(big, big, om_big, "cavity", 0.1, fmt=name, variant="auto")
             steps = 200
@@ -312,10 +356,22 @@ def main():
             m = big * big * steps / s / 1e6
             gbs = BYTES_PER_LUP[name] * m * 1e6 / 1e9
             legs[name] = {"mlups": round(m, 1), "steps": steps, "variant": lat.variant,
+                          "steps_per_launch": lat.multi_k,
                           "achieved_gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4)}
             del lat
             torch.cuda.empty_cache()
+        lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant,
+                      multi_step=False)
+        s = timed_steps(lat, 200, 20)
+        del lat
+        torch.cuda.empty_cache()
+        m = big * big * 200 / s / 1e6
+        legs["f64_single_step"] = {"mlups": round(m, 1), "achieved_gbs": round(144 * m / 1e3, 1),
+                                   "frac_of_8tbs": round(144 * m / 1e3 / HBM_PEAK_GBS, 4)}
+        # FP16 storage against FP64 with temporal blocking (the fastest FP64
+        # form) and against the plain single-pass FP64 kernel
         legs["f16_over_f64_mlups"] = round(legs["f16"]["mlups"] / legs["f64"]["mlups"], 3)
+        legs["f16_over_f64_single_step_mlups"] = round(legs["f16"]["mlups"] / m, 3)
         extra["cavity_8192"] = legs
         nbytes = 9 * big * big * 8
         extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
@@ -337,7 +393,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(launch_s * 1e3, 5),
+        "ms_per_step": round(sec / args.steps * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -348,7 +404,7 @@ def main():
                    "nx": n, "ny": n, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": profiled_traffic("step_f64_cavity_%d_traffic.json" % n)},
+                     "traffic": traffic},
         "cpu_baseline": None,
         "extra": extra,
     }

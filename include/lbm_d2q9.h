@@ -87,21 +87,23 @@ int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *r
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
                         int elem_bytes, void *stream);
 
-/* Two full time steps src -> dst in one launch (temporal blocking): each
- * block computes the first step on its tile and a one-node halo into LDS and
- * the second from there, so HBM sees about one read and one write of each
- * population per two steps.  Bitwise equal to two lbm_step_layout_* calls
- * (same arithmetic, cavity_opt1.py:341-343 twice); rho/ux/uy (nullable, all
- * or none) receive the pre-collision moments of the second step.  variant
- * picks a tile shape (0 .. lbm_step2_variant_count()-1, -1 default); the grid
- * must be at least lbm_step2_min_size(variant) (LBM_E_SIZE otherwise).        */
-int lbm_step2_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout, double omega,
-                  int bc, double u0, double *rho, double *ux, double *uy, int variant,
-                  void *stream);
-int lbm_step2_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout, float omega,
-                  int bc, float u0, float *rho, float *ux, float *uy, int variant, void *stream);
-int lbm_step2_variant_count(void);
-int lbm_step2_min_size(int variant, int64_t *min_nx, int64_t *min_ny);
+/* K full time steps src -> dst in one launch (temporal blocking, K = 2..4
+ * fixed by the variant): each block computes the first K-1 steps of its tile
+ * and a shrinking halo in LDS and writes only the last, so HBM sees about one
+ * read and one write of each population per K steps.  Bitwise equal to K
+ * lbm_step_layout_* calls (same arithmetic, cavity_opt1.py:341-343 K times);
+ * rho/ux/uy (nullable, all or none) receive the pre-collision moments of the
+ * last step.  variant picks (K, tile shape), 0 .. lbm_step_multi_variant_count()-1
+ * or -1 for the default; lbm_step_multi_info gives its K and the smallest grid
+ * it takes (LBM_E_SIZE for smaller grids).                                   */
+int lbm_step_multi_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout,
+                       double omega, int bc, double u0, double *rho, double *ux, double *uy,
+                       int variant, void *stream);
+int lbm_step_multi_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout,
+                       float omega, int bc, float u0, float *rho, float *ux, float *uy,
+                       int variant, void *stream);
+int lbm_step_multi_variant_count(void);
+int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 
 /* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with
  * 16-byte loads and stores, non-temporal stores if nontemporal != 0.  No
@@ -206,6 +208,18 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
                  float omega, int bc, float u0, float scale, int compute, int rounding,
                  uint32_t seed, float *ux, float *uy, int variant, void *stream);
+/* K steps src -> dst in one launch (temporal blocking, K = 2..4 fixed by the
+ * variant), as lbm_step_multi_* for stored deviations: LDS holds the stored
+ * values, so step j of the launch (j = 0..K-1) rounds exactly as a
+ * lbm_dev_step call with seed + j * seed_stride (mod 2^32), and the result is
+ * bitwise that of those K calls.  ux/uy receive the last step's velocity.
+ * lbm_dev_multi_info gives the variant's K and the smallest grid it takes.    */
+int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
+                       float omega, int bc, float u0, float scale, int compute, int rounding,
+                       uint32_t seed, uint32_t seed_stride, float *ux, float *uy, int variant,
+                       void *stream);
+int lbm_dev_multi_variant_count(void);
+int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 /* *out = sum_{i,k,l} f = nx*ny + (sum of stored values) / scale, in FP64     */
 int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,
                  double *out, void *stream);

@@ -66,12 +66,13 @@ SIGNATURES = {
     "lbm_moments_f64": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
     "lbm_moments_f32": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
     "lbm_copy": (_i32, [_vp, _vp, _i64, _i32, _vp]),
-    "lbm_step2_f64": (_i32, [_vp, _vp, _i64, _i64, _i32, _f64, _i32, _f64, _vp, _vp, _vp, _i32,
-                             _vp]),
-    "lbm_step2_f32": (_i32, [_vp, _vp, _i64, _i64, _i32, _f32, _i32, _f32, _vp, _vp, _vp, _i32,
-                             _vp]),
-    "lbm_step2_variant_count": (_i32, []),
-    "lbm_step2_min_size": (_i32, [_i32, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "lbm_step_multi_f64": (_i32, [_vp, _vp, _i64, _i64, _i32, _f64, _i32, _f64, _vp, _vp, _vp,
+                                  _i32, _vp]),
+    "lbm_step_multi_f32": (_i32, [_vp, _vp, _i64, _i64, _i32, _f32, _i32, _f32, _vp, _vp, _vp,
+                                  _i32, _vp]),
+    "lbm_step_multi_variant_count": (_i32, []),
+    "lbm_step_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
+                                   ctypes.POINTER(_i64)]),
     "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),
     "lbm_stream_periodic": (_i32, [_vp, _vp, _i64, _i64, _i32, _vp]),
     "lbm_stream_bounce_back_f64": (_i32, [_vp, _vp, _i64, _i64, _f64, _vp]),
@@ -92,6 +93,12 @@ SIGNATURES = {
     "lbm_dev_decode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
     "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32, _i32,
                             _i32, ctypes.c_uint32, _vp, _vp, _i32, _vp]),
+    "lbm_dev_step_multi": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32,
+                                  _i32, _i32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _i32,
+                                  _vp]),
+    "lbm_dev_multi_variant_count": (_i32, []),
+    "lbm_dev_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
+                                  ctypes.POINTER(_i64)]),
     "lbm_dev_mass": (_i32, [_vp, _i64, _i64, _i32, _f64, _vp, _vp, _vp]),
     "lbm_dev_velocity_residual": (_i32, [_vp, _i64, _i64, _i32, _i32, _f64, _vp, _vp, _vp, _vp,
                                          _vp]),

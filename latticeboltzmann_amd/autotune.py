@@ -72,11 +72,12 @@ def time_variants(launch, variants, reps=10, warmup=2):
 
 
 def best_variant(kernel, nx, ny, dtype, layout, bc, make_launch, variants, reps=10,
-                 use_cache=True):
+                 use_cache=True, normalize=None):
     """Fastest variant of ``kernel`` for this problem class.
 
     ``make_launch(nx, ny)`` returns a callable ``launch(variant)`` that runs one
-    step on scratch buffers of that size (it owns them).
+    launch on scratch buffers of that size (it owns them); ``normalize`` maps
+    the launch times to the times compared (e.g. per step).
     """
     dev = torch.cuda.get_device_name(torch.cuda.current_device())
     key = "|".join([dev, kernel, str(dtype), str(layout), str(bc), size_class(nx, ny)])
@@ -89,6 +90,8 @@ def best_variant(kernel, nx, ny, dtype, layout, bc, make_launch, variants, reps=
             return _MEMO[key]
     launch = make_launch(nx, ny)
     times = time_variants(launch, list(variants), reps=reps)
+    if normalize is not None:
+        times = normalize(times)
     best = min(times, key=times.get)
     with _LOCK:
         _MEMO[key] = best
@@ -120,5 +123,41 @@ def step_variant(nx, ny, dtype, layout, bc, reps=10, use_cache=True):
         return launch
 
     n_var = nat.load().lbm_step_variant_count()
-    return best_variant("step", nx, ny, dtype, layout, bc, make_launch, range(n_var), reps=reps,
-                        use_cache=use_cache)
+    return best_variant("step%d" % n_var, nx, ny, dtype, layout, bc, make_launch, range(n_var),
+                        reps=reps, use_cache=use_cache)
+
+
+def multi_variant(nx, ny, dtype, layout, bc, reps=10, use_cache=True):
+    """Autotuned variant of the multi-step kernel (``lbm_step_multi_*``): the
+    (steps per launch, tile) shape with the least time per step, or None when
+    the grid is smaller than every shape needs."""
+    from . import kernels
+    n_var = nat.load().lbm_step_multi_variant_count()
+    info = {v: kernels.step_multi_info(v) for v in range(n_var)}
+    fits = [v for v, (k, mnx, mny) in info.items() if nx >= mnx and ny >= mny]
+    if not fits:
+        return None
+
+    def make_launch(nx, ny):
+        shape = (9, nx, ny) if layout == nat.LAYOUT_SOA else (nx, 9, ny)
+        a = torch.empty(shape, dtype=dtype, device="cuda")
+        rows = torch.ones((nx, ny), dtype=dtype, device="cuda")
+        zero = torch.zeros_like(rows)
+        soa = kernels.equilibrium(rows, zero, zero)
+        if layout == nat.LAYOUT_SOA:
+            a.copy_(soa)
+        else:
+            kernels.relayout(soa, nat.LAYOUT_SOA, nat.LAYOUT_ROWS, out=a)
+        b = torch.empty_like(a)
+
+        def launch(v):
+            kernels.step_multi(a, b, 1.0, bc, 0.1, variant=v, layout=layout)
+        return launch
+
+    def per_step(times):
+        return {v: t / info[v][0] for v, t in times.items()}
+
+    # the table size in the key keeps a cache written for another shape table
+    # from naming the wrong shape
+    return best_variant("step_multi%d" % n_var, nx, ny, dtype, layout, bc, make_launch, fits,
+                        reps=reps, use_cache=use_cache, normalize=per_step)

@@ -69,4 +69,31 @@ __device__ __forceinline__ void pull9(const PullRows<T> &r, int l, int ny, T g[9
     g[8] = ld<NT>(r.p[8] + lp);
 }
 
+// Pulls of node m of a tile held in LDS as planes of N values, rows W wide
+// (the multi-step kernels): channel i from the neighbour at -c_i.
+template <typename T>
+__device__ __forceinline__ void pull_lds(const T *in, int N, int W, int m, T g[9]) {
+    g[0] = in[m];
+    g[1] = in[1 * N + m - W];      // E  from (k-1, l)
+    g[2] = in[2 * N + m - 1];      // N  from (k,   l-1)
+    g[3] = in[3 * N + m + W];      // W  from (k+1, l)
+    g[4] = in[4 * N + m + 1];      // S  from (k,   l+1)
+    g[5] = in[5 * N + m - W - 1];  // NE from (k-1, l-1)
+    g[6] = in[6 * N + m + W - 1];  // NW from (k+1, l-1)
+    g[7] = in[7 * N + m + W + 1];  // SW from (k+1, l+1)
+    g[8] = in[8 * N + m - W + 1];  // SE from (k-1, l+1)
+}
+
+// Tile of this block for the multi-step kernels, numbered so that the blocks
+// on one XCD (the hardware deals blocks to the 8 XCDs round-robin) take a
+// contiguous band of tiles, whose halos then meet in that XCD's L2.  The
+// grid holds ceil(ntiles / 8) * 8 blocks; a result >= ntiles means no tile.
+__device__ __forceinline__ int xcd_tile(int ntiles) {
+    const int per = (ntiles + 7) >> 3;
+    return int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
+}
+
+// Wrap a coordinate that is at most one period outside [0, n).
+__device__ __forceinline__ int wrap1(int k, int n) { return k < 0 ? k + n : (k >= n ? k - n : k); }
+
 }  // namespace lbm

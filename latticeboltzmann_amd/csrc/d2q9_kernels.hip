@@ -105,34 +105,28 @@ __global__ __launch_bounds__(BS) void k_step(const T *__restrict__ src, T *__res
     }
 }
 
-// Two fused steps per launch (temporal blocking), src -> dst.
+// K fused steps per launch (temporal blocking, K = 2..4), src -> dst.
 // Block b owns a TR x TC tile of nodes.  Phase 1 computes step 1 on the tile
-// and a one-node halo ring, (TR + 2) x (TC + 2) nodes at wrapped global
-// coordinates, with the single-step node code (pull from src, walls, collide),
-// into LDS.  Phase 2 computes step 2 of the tile from LDS and writes dst.  The
-// wrapped halo also supplies the x-wrapped lid densities of the two top
-// corners (cavity_opt1.py:206-213), so the result is bitwise that of two
-// k_step launches; HBM sees one read and one write per population per two
-// steps plus the halo re-reads that L2 does not absorb.  Tiles are numbered so
-// that the blocks on one XCD (blockIdx % 8) take a contiguous band of tiles,
-// whose halos then meet in that XCD's L2.  Requires nx >= TR + 2, ny >= TC + 2.
-template <typename T, int BC, bool MOM, int TR, int TC>
-__global__ __launch_bounds__(kBlock) void k_step2(const T *__restrict__ src, T *__restrict__ dst,
-                                                  int nx, int ny, Layout L, int ntc, int ntiles,
-                                                  T omega, T u0, T *__restrict__ rho_o,
-                                                  T *__restrict__ ux_o, T *__restrict__ uy_o) {
-    constexpr int HR = TR + 2, HC = TC + 2, HN = HR * HC;
-    __shared__ T s[9 * HN];
-    // XCD-contiguous tile order: blocks land on XCD blockIdx % 8 in turn
-    const int per = (ntiles + 7) >> 3;
-    const int tile = int(blockIdx.x & 7) * per + int(blockIdx.x >> 3);
-    if (tile >= ntiles) return;  // whole block: before any barrier
-    const int tr = tile / ntc, tc = tile - tr * ntc;
-    const int k0 = tr * TR, l0 = tc * TC;
+// and a halo ring K-1 nodes deep, at wrapped global coordinates, with the
+// single-step node code (pull from src, walls, collide), into LDS; each later
+// phase computes the next step on a ring one node shallower from the previous
+// phase's LDS values (two LDS buffers, used in turn); the last phase writes the
+// tile to dst.  The wrapped halo also supplies the x-wrapped lid densities of
+// the two top corners (cavity_opt1.py:206-213), so the result is bitwise that
+// of K single-step launches, while HBM sees one read and one write per
+// population per K steps plus the halo re-reads that L2 does not absorb.
+// Tiles are numbered so that the blocks on one XCD (blockIdx % 8) take a
+// contiguous band of tiles, whose halos then meet in that XCD's L2.
+// Requires nx >= TR + 2K, ny >= TC + 2K.
 
-    for (int n = threadIdx.x; n < HN; n += kBlock) {
-        const int r = n / HC, c = n - r * HC;
-        int k = k0 - 1 + r, l = l0 - 1 + c;
+// One phase from global memory: step of the nodes of a halo-H tile ring.
+template <typename T, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *out, int k0, int l0,
+                                                  int nx, int ny, Layout L, T omega, T u0) {
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        int k = k0 - H + r, l = l0 - H + c;
         k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
         l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
         const PullRows<T> rows(src, L, k, nx);
@@ -143,26 +137,47 @@ __global__ __launch_bounds__(kBlock) void k_step2(const T *__restrict__ src, T *
         T rho, ux, uy;
         collide_opt1(g, omega, rho, ux, uy);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) s[i * HN + n] = g[i];
+        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
     }
-    __syncthreads();
+}
 
-    for (int n = threadIdx.x; n < TR * TC; n += kBlock) {
+// One phase from LDS to LDS: halo-(H+1) values in, halo-H values out.
+template <typename T, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void phase_lds(const T *in, T *out, int k0, int l0, int nx, int ny,
+                                          T omega, T u0) {
+    constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        int k = k0 - H + r, l = l0 - H + c;
+        k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
+        l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
+        const int m = (r + 1) * Wi + (c + 1);
+        T g[9];
+        pull_lds(in, Ni, Wi, m, g);
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
+        T rho, ux, uy;
+        collide_opt1(g, omega, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+    }
+}
+
+// Last phase: the tile from halo-1 LDS values to dst.
+template <typename T, int BC, bool MOM, int TR, int TC, int BS>
+__device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst, int k0, int l0,
+                                                int nx, int ny, Layout L, T omega, T u0,
+                                                T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                                T *__restrict__ uy_o) {
+    constexpr int Wi = TC + 2, Ni = (TR + 2) * Wi;
+    for (int n = threadIdx.x; n < TR * TC; n += BS) {
         const int r = n / TC, c = n - r * TC;
         const int k = k0 + r, l = l0 + c;
         if (k >= nx || l >= ny) continue;
-        const int m = (r + 1) * HC + (c + 1);
+        const int m = (r + 1) * Wi + (c + 1);
         T g[9];
-        g[0] = s[m];
-        g[1] = s[1 * HN + m - HC];      // E  from (k-1, l)
-        g[2] = s[2 * HN + m - 1];       // N  from (k,   l-1)
-        g[3] = s[3 * HN + m + HC];      // W  from (k+1, l)
-        g[4] = s[4 * HN + m + 1];       // S  from (k,   l+1)
-        g[5] = s[5 * HN + m - HC - 1];  // NE from (k-1, l-1)
-        g[6] = s[6 * HN + m + HC - 1];  // NW from (k+1, l-1)
-        g[7] = s[7 * HN + m + HC + 1];  // SW from (k+1, l+1)
-        g[8] = s[8 * HN + m - HC + 1];  // SE from (k-1, l+1)
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g, s + m, int64_t(HN), k, l, nx, ny, u0);
+        pull_lds(in, Ni, Wi, m, g);
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
         T rho, ux, uy;
         collide_opt1(g, omega, rho, ux, uy);
         T *d = dst + int64_t(k) * L.rs + l;
@@ -174,6 +189,38 @@ __global__ __launch_bounds__(kBlock) void k_step2(const T *__restrict__ src, T *
             ux_o[j] = ux;
             uy_o[j] = uy;
         }
+    }
+}
+
+template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
+__global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__restrict__ dst,
+                                                  int nx, int ny, Layout L, int ntc, int ntiles,
+                                                  T omega, T u0, T *__restrict__ rho_o,
+                                                  T *__restrict__ ux_o, T *__restrict__ uy_o) {
+    static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));  // halo K-1 (and K-3)
+    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;  // halo K-2
+    __shared__ T sa[9 * NA];
+    __shared__ T sb[9 * NB];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // whole block: before any barrier
+    const int tr = tile / ntc, tc = tile - tr * ntc;
+    const int k0 = tr * TR, l0 = tc * TC;
+
+    phase_from_global<T, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0);
+    __syncthreads();
+    if constexpr (K == 2) {
+        phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
+    } else if constexpr (K == 3) {
+        phase_lds<T, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        phase_to_global<T, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
+    } else {
+        phase_lds<T, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        phase_lds<T, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
     }
 }
 
@@ -428,82 +475,88 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
     return launch_status();
 }
 
-// Tile shapes of the two-step kernel, indexed by `variant`: (rows TR, columns TC).
-#define LBM_STEP2_SHAPES(X) \
-    X(0, 8, 64)             \
-    X(1, 4, 128)            \
-    X(2, 16, 32)            \
-    X(3, 4, 64)
-#define LBM_COUNT_SHAPE2(id, tr, tc) +1
-constexpr int kNumStep2Shapes = 0 LBM_STEP2_SHAPES(LBM_COUNT_SHAPE2);
-constexpr int kDefaultStep2Shape = 0;
+// Shapes of the multi-step kernel, indexed by `variant`: (steps per launch K,
+// tile rows TR, tile columns TC, threads per block BS).
+#define LBM_MULTI_SHAPES(X)   \
+    X(0, 2, 8, 64, 256)       \
+    X(1, 2, 4, 128, 256)      \
+    X(2, 2, 16, 32, 256)      \
+    X(3, 2, 4, 64, 256)       \
+    X(4, 3, 8, 32, 256)       \
+    X(5, 3, 4, 64, 256)       \
+    X(6, 4, 8, 32, 256)       \
+    X(7, 2, 8, 64, 512)       \
+    X(8, 2, 8, 128, 512)      \
+    X(9, 3, 8, 32, 512)       \
+    X(10, 3, 12, 32, 512)     \
+    X(11, 3, 8, 64, 512)      \
+    X(12, 4, 8, 32, 512)
+#define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
+constexpr int kNumMultiShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
+// Three steps on 12 x 32 tiles with 512 threads: the fastest FP64 shape measured
+// (83 GLUPS at 2048^2, 89 at 8192^2; 75.8 KB of LDS, so two blocks per CU).
+constexpr int kDefaultMultiShape = 10;
 
-template <int TR, int TC>
-bool step2_fits(int64_t nx, int64_t ny) {
-    return nx >= TR + 2 && ny >= TC + 2;
-}
-
-template <typename T, int BC, bool MOM, int TR, int TC>
-int launch_step2_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
+template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
+int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
                    T *ux, T *uy, hipStream_t s) {
-    if (!step2_fits<TR, TC>(nx, ny)) return LBM_E_SIZE;
+    if (nx < TR + 2 * K || ny < TC + 2 * K) return LBM_E_SIZE;
     const int64_t ntr = (nx + TR - 1) / TR, ntc = (ny + TC - 1) / TC;
     const int64_t ntiles = ntr * ntc;
     const int64_t nblocks = ((ntiles + 7) / 8) * 8;
     if (nblocks > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL((k_step2<T, BC, MOM, TR, TC>), dim3(unsigned(nblocks)), dim3(kBlock), 0, s,
-                       src, dst, int(nx), int(ny), L, int(ntc), int(ntiles), omega, u0, rho, ux,
+    hipLaunchKernelGGL((k_stepk<T, BC, MOM, K, TR, TC, BS>), dim3(unsigned(nblocks)), dim3(BS), 0,
+                       s, src, dst, int(nx), int(ny), L, int(ntc), int(ntiles), omega, u0, rho, ux,
                        uy);
     return LBM_OK;
 }
 
 template <typename T, int BC, bool MOM>
-int launch_step2_bc(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
+int launch_multi_bc(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
                     T *ux, T *uy, int shape, hipStream_t s) {
-#define LBM_LAUNCH_SHAPE2(id, tr, tc) \
-    case id: return launch_step2_v<T, BC, MOM, tr, tc>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s);
+#define LBM_LAUNCH_MULTI(id, k, tr, tc, bs) \
+    case id: return launch_multi_v<T, BC, MOM, k, tr, tc, bs>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s);
     switch (shape) {
-        LBM_STEP2_SHAPES(LBM_LAUNCH_SHAPE2)
+        LBM_MULTI_SHAPES(LBM_LAUNCH_MULTI)
         default: return LBM_E_ARG;
     }
-#undef LBM_LAUNCH_SHAPE2
+#undef LBM_LAUNCH_MULTI
 }
 
 template <typename T>
-int step2_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega, int bc, T u0,
+int multi_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega, int bc, T u0,
                T *rho, T *ux, T *uy, int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if (!valid_grid(nx, ny) || nx > INT32_MAX || ny > INT32_MAX) return LBM_E_SIZE;
     if (!valid_layout(layout)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
-    if (variant < -1 || variant >= kNumStep2Shapes) return LBM_E_ARG;
+    if (variant < -1 || variant >= kNumMultiShapes) return LBM_E_ARG;
     const bool mom = rho != nullptr;
     if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
     const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int shape = variant < 0 ? kDefaultStep2Shape : variant;
+    const int shape = variant < 0 ? kDefaultMultiShape : variant;
     hipStream_t s = as_stream(stream);
     const Layout L = make_layout(layout, nx, ny);
     int rc;
     if (bc == LBM_BC_CAVITY)
-        rc = mom ? launch_step2_bc<T, LBM_BC_CAVITY, true>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s)
-                 : launch_step2_bc<T, LBM_BC_CAVITY, false>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s);
+        rc = mom ? launch_multi_bc<T, LBM_BC_CAVITY, true>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s)
+                 : launch_multi_bc<T, LBM_BC_CAVITY, false>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s);
     else
-        rc = mom ? launch_step2_bc<T, LBM_BC_PERIODIC, true>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s)
-                 : launch_step2_bc<T, LBM_BC_PERIODIC, false>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s);
+        rc = mom ? launch_multi_bc<T, LBM_BC_PERIODIC, true>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s)
+                 : launch_multi_bc<T, LBM_BC_PERIODIC, false>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, shape, s);
     return rc ? rc : launch_status();
 }
 
-template <typename T>
-int step2_min_size(int variant, int64_t *min_nx, int64_t *min_ny) {
-    const int shape = variant < 0 ? kDefaultStep2Shape : variant;
-#define LBM_MIN_SHAPE2(id, tr, tc) \
-    case id: *min_nx = tr + 2; *min_ny = tc + 2; return LBM_OK;
+inline int multi_shape_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
+    const int shape = variant < 0 ? kDefaultMultiShape : variant;
+#define LBM_INFO_MULTI(id, k, tr, tc, bs) \
+    case id: *steps = k; *min_nx = tr + 2 * k; *min_ny = tc + 2 * k; return LBM_OK;
     switch (shape) {
-        LBM_STEP2_SHAPES(LBM_MIN_SHAPE2)
+        LBM_MULTI_SHAPES(LBM_INFO_MULTI)
         default: return LBM_E_ARG;
     }
-#undef LBM_MIN_SHAPE2
+#undef LBM_INFO_MULTI
 }
 
 template <typename T>
@@ -608,11 +661,11 @@ extern "C" {
 const char *lbm_version(void) { return "latticeboltzmann_amd 0.1.0 gfx950"; }
 int lbm_step_variant_count(void) { return kNumStepShapes; }
 
-int lbm_step2_variant_count(void) { return kNumStep2Shapes; }
+int lbm_step_multi_variant_count(void) { return kNumMultiShapes; }
 
-int lbm_step2_min_size(int variant, int64_t *min_nx, int64_t *min_ny) {
-    if (!min_nx || !min_ny) return LBM_E_NULL;
-    return step2_min_size<double>(variant, min_nx, min_ny);
+int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
+    if (!steps || !min_nx || !min_ny) return LBM_E_NULL;
+    return multi_shape_info(variant, steps, min_nx, min_ny);
 }
 
 int lbm_equilibrium_f64(const double *rho, const double *ux, const double *uy, double *f,
@@ -680,15 +733,16 @@ int lbm_copy(const void *src, void *dst, int64_t nbytes, int nontemporal, void *
     return launch_status();
 }
 
-int lbm_step2_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout, double omega,
-                  int bc, double u0, double *rho, double *ux, double *uy, int variant,
-                  void *stream) {
-    return step2_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
+int lbm_step_multi_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout,
+                       double omega, int bc, double u0, double *rho, double *ux, double *uy,
+                       int variant, void *stream) {
+    return multi_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
 }
 
-int lbm_step2_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout, float omega,
-                  int bc, float u0, float *rho, float *ux, float *uy, int variant, void *stream) {
-    return step2_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
+int lbm_step_multi_f32(const float *src, float *dst, int64_t nx, int64_t ny, int layout,
+                       float omega, int bc, float u0, float *rho, float *ux, float *uy,
+                       int variant, void *stream) {
+    return multi_impl(src, dst, nx, ny, layout, omega, bc, u0, rho, ux, uy, variant, stream);
 }
 
 int lbm_relayout(const void *src, void *dst, int64_t nx, int64_t ny, int elem_bytes,

@@ -70,6 +70,158 @@ __global__ __launch_bounds__(BS) void k_dev_step(const typename Codec<FMT>::raw 
     }
 }
 
+// K fused steps per launch on stored deviations (temporal blocking, K = 2..4),
+// the structure of k_stepk: phase 1 computes the first step of the tile and a
+// halo ring K-1 deep from src into LDS, each later phase the next step on a
+// ring one node shallower, the last phase writes the tile to dst.  LDS holds
+// the stored (rounded) values, so every intermediate step is rounded, with the
+// dither of its own step (seed + j * seed_stride for step j of the launch) and
+// node, exactly as a single-step launch rounds it: the result is bitwise that
+// of K k_dev_step launches.  Requires nx >= TR + 2K, ny >= TC + 2K.
+template <int FMT, typename CT, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void dev_phase_global(const typename Codec<FMT>::raw *__restrict__ src,
+                                                 typename Codec<FMT>::raw *out, int k0, int l0,
+                                                 int nx, int ny, Layout L, CT omega, CT u0,
+                                                 CT scale, CT inv_scale, int stochastic,
+                                                 uint32_t seed) {
+    using C = Codec<FMT>;
+    using R = typename C::raw;
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+        const PullRows<R> rows(src, L, k, nx);
+        R raw[9];
+        pull9<false>(rows, l, ny, raw);
+        CT g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
+                                      inv_scale);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i * N + n] = raw[i];
+    }
+}
+
+template <int FMT, typename CT, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void dev_phase_lds(const typename Codec<FMT>::raw *in,
+                                              typename Codec<FMT>::raw *out, int k0, int l0,
+                                              int nx, int ny, CT omega, CT u0, CT scale,
+                                              CT inv_scale, int stochastic, uint32_t seed) {
+    using C = Codec<FMT>;
+    using R = typename C::raw;
+    constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+        const int m = (r + 1) * Wi + (c + 1);
+        R raw[9];
+        pull_lds(in, Ni, Wi, m, raw);
+        CT g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls_dev<FMT, CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0, inv_scale);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i * N + n] = raw[i];
+    }
+}
+
+template <int FMT, typename CT, int BC, bool MOM, int TR, int TC, int BS>
+__device__ __forceinline__ void dev_phase_to_global(const typename Codec<FMT>::raw *in,
+                                                    typename Codec<FMT>::raw *__restrict__ dst,
+                                                    int k0, int l0, int nx, int ny, Layout L,
+                                                    CT omega, CT u0, CT scale, CT inv_scale,
+                                                    int stochastic, uint32_t seed,
+                                                    float *__restrict__ ux_o,
+                                                    float *__restrict__ uy_o) {
+    using C = Codec<FMT>;
+    using R = typename C::raw;
+    constexpr int Wi = TC + 2, Ni = (TR + 2) * Wi;
+    for (int n = threadIdx.x; n < TR * TC; n += BS) {
+        const int r = n / TC, c = n - r * TC;
+        const int k = k0 + r, l = l0 + c;
+        if (k >= nx || l >= ny) continue;
+        const int m = (r + 1) * Wi + (c + 1);
+        R raw[9];
+        pull_lds(in, Ni, Wi, m, raw);
+        CT g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls_dev<FMT, CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0, inv_scale);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+        R *d = dst + int64_t(k) * L.rs + l;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) d[i * L.cs] = raw[i];
+        if constexpr (MOM) {
+            const int64_t j = int64_t(k) * ny + l;
+            ux_o[j] = float(ux);
+            uy_o[j] = float(uy);
+        }
+    }
+}
+
+template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
+__global__ __launch_bounds__(BS) void k_dev_stepk(const typename Codec<FMT>::raw *__restrict__ src,
+                                                  typename Codec<FMT>::raw *__restrict__ dst,
+                                                  int nx, int ny, Layout L, int ntc, int ntiles,
+                                                  CT omega, CT u0, CT scale, CT inv_scale,
+                                                  int stochastic, uint32_t seed,
+                                                  uint32_t seed_stride, float *__restrict__ ux_o,
+                                                  float *__restrict__ uy_o) {
+    using R = typename Codec<FMT>::raw;
+    static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
+    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
+    __shared__ R sa[9 * NA];
+    __shared__ R sb[9 * NB];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // whole block: before any barrier
+    const int tr = tile / ntc, tc = tile - tr * ntc;
+    const int k0 = tr * TR, l0 = tc * TC;
+    const uint32_t s1 = seed + seed_stride, s2 = s1 + seed_stride, s3 = s2 + seed_stride;
+
+    dev_phase_global<FMT, CT, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0, scale,
+                                                     inv_scale, stochastic, seed);
+    __syncthreads();
+    if constexpr (K == 2) {
+        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
+                                                          scale, inv_scale, stochastic, s1, ux_o,
+                                                          uy_o);
+    } else if constexpr (K == 3) {
+        dev_phase_lds<FMT, CT, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0, scale,
+                                                  inv_scale, stochastic, s1);
+        __syncthreads();
+        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0,
+                                                          scale, inv_scale, stochastic, s2, ux_o,
+                                                          uy_o);
+    } else {
+        dev_phase_lds<FMT, CT, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0, scale,
+                                                  inv_scale, stochastic, s1);
+        __syncthreads();
+        dev_phase_lds<FMT, CT, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0, scale,
+                                                  inv_scale, stochastic, s2);
+        __syncthreads();
+        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
+                                                          scale, inv_scale, stochastic, s3, ux_o,
+                                                          uy_o);
+    }
+}
+
 // FP64 populations (reference layout) -> stored deviations (any layout).
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_dev_encode(const double *__restrict__ f,
@@ -229,6 +381,74 @@ void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc
     }
 }
 
+// Shapes of the multi-step deviation kernel, indexed by `variant`: (steps per
+// launch K, tile rows TR, tile columns TC, threads per block BS).
+#define LBM_DEV_MULTI_SHAPES(X) \
+    X(0, 2, 8, 64, 256)         \
+    X(1, 2, 16, 64, 512)        \
+    X(2, 3, 12, 32, 512)        \
+    X(3, 3, 16, 64, 512)        \
+    X(4, 4, 8, 64, 512)         \
+    X(5, 4, 16, 64, 512)
+#define LBM_DEV_COUNT_MULTI(id, k, tr, tc, bs) +1
+constexpr int kNumDevMultiShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
+constexpr int kDefaultDevMultiShape = 2;
+
+template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
+int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t seed_stride,
+                       hipStream_t s) {
+    using R = typename Codec<FMT>::raw;
+    if (a.nx < TR + 2 * K || a.ny < TC + 2 * K) return LBM_E_SIZE;
+    const int64_t ntr = (a.nx + TR - 1) / TR, ntc = (a.ny + TC - 1) / TC;
+    const int64_t ntiles = ntr * ntc;
+    const int64_t nblocks = ((ntiles + 7) / 8) * 8;
+    if (nblocks > 0x7fffffffLL) return LBM_E_SIZE;
+    hipLaunchKernelGGL((k_dev_stepk<FMT, CT, BC, MOM, K, TR, TC, BS>), dim3(unsigned(nblocks)),
+                       dim3(BS), 0, s, static_cast<const R *>(src), static_cast<R *>(dst), a.nx,
+                       a.ny, a.L, int(ntc), int(ntiles), CT(a.omega), CT(a.u0), CT(a.scale),
+                       CT(1) / CT(a.scale), a.stochastic, a.seed, seed_stride, a.ux, a.uy);
+    return LBM_OK;
+}
+
+template <int FMT, typename CT, int BC, bool MOM>
+int launch_dev_multi_bc(const void *src, void *dst, const DevArgs &a, uint32_t stride, int shape,
+                        hipStream_t s) {
+#define LBM_DEV_LAUNCH_MULTI(id, k, tr, tc, bs) \
+    case id: return launch_dev_multi_v<FMT, CT, BC, MOM, k, tr, tc, bs>(src, dst, a, stride, s);
+    switch (shape) {
+        LBM_DEV_MULTI_SHAPES(LBM_DEV_LAUNCH_MULTI)
+        default: return LBM_E_ARG;
+    }
+#undef LBM_DEV_LAUNCH_MULTI
+}
+
+template <int FMT, typename CT>
+int launch_dev_multi_fmt(const void *src, void *dst, int bc, const DevArgs &a, uint32_t stride,
+                         int shape, hipStream_t s) {
+    const bool mom = a.ux != nullptr;
+    if (bc == LBM_BC_CAVITY)
+        return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, true>(src, dst, a, stride, shape, s)
+                   : launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, false>(src, dst, a, stride, shape, s);
+    return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, true>(src, dst, a, stride, shape, s)
+               : launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, false>(src, dst, a, stride, shape, s);
+}
+
+int launch_dev_multi(const void *src, void *dst, int fmt, int compute, int bc, const DevArgs &a,
+                     uint32_t stride, int shape, hipStream_t s) {
+    if (compute == LBM_COMPUTE_F64) {
+        switch (fmt) {
+            case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, double>(src, dst, bc, a, stride, shape, s);
+            case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, double>(src, dst, bc, a, stride, shape, s);
+            default: return launch_dev_multi_fmt<LBM_FMT_F16, double>(src, dst, bc, a, stride, shape, s);
+        }
+    }
+    switch (fmt) {
+        case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, float>(src, dst, bc, a, stride, shape, s);
+        case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, float>(src, dst, bc, a, stride, shape, s);
+        default: return launch_dev_multi_fmt<LBM_FMT_F16, float>(src, dst, bc, a, stride, shape, s);
+    }
+}
+
 inline int64_t dev_blocks(int64_t nx, int64_t ny) { return 9 * nx * ((ny + kBlock - 1) / kBlock); }
 
 }  // namespace lbm
@@ -315,6 +535,42 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
     hipStream_t s = as_stream(stream);
     launch_dev_compute(src, dst, fmt, compute, bc, a, shape, s);
     return launch_status();
+}
+
+int lbm_dev_multi_variant_count(void) { return kNumDevMultiShapes; }
+
+int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
+    if (!steps || !min_nx || !min_ny) return LBM_E_NULL;
+    const int shape = variant < 0 ? kDefaultDevMultiShape : variant;
+#define LBM_DEV_INFO_MULTI(id, k, tr, tc, bs) \
+    case id: *steps = k; *min_nx = tr + 2 * k; *min_ny = tc + 2 * k; return LBM_OK;
+    switch (shape) {
+        LBM_DEV_MULTI_SHAPES(LBM_DEV_INFO_MULTI)
+        default: return LBM_E_ARG;
+    }
+#undef LBM_DEV_INFO_MULTI
+}
+
+int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
+                       float omega, int bc, float u0, float scale, int compute, int rounding,
+                       uint32_t seed, uint32_t seed_stride, float *ux, float *uy, int variant,
+                       void *stream) {
+    if (!src || !dst) return LBM_E_NULL;
+    if ((ux == nullptr) != (uy == nullptr)) return LBM_E_NULL;
+    if (!valid_grid(nx, ny) || nx > INT32_MAX || ny > INT32_MAX) return LBM_E_SIZE;
+    if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0f)) return LBM_E_ARG;
+    if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
+    if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
+    if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
+    if (variant < -1 || variant >= kNumDevMultiShapes) return LBM_E_ARG;
+    const int64_t bytes = 9 * nx * ny * fmt_bytes(fmt);
+    if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
+    const int shape = variant < 0 ? kDefaultDevMultiShape : variant;
+    const DevArgs a{int(nx), int(ny), make_layout(layout, nx, ny), omega, u0, scale,
+                    rounding == LBM_ROUND_STOCHASTIC ? 1 : 0, seed, ux, uy};
+    const int rc = launch_dev_multi(src, dst, fmt, compute, bc, a, seed_stride, shape,
+                                    as_stream(stream));
+    return rc ? rc : launch_status();
 }
 
 int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,

@@ -172,36 +172,37 @@ def step(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
     return dst
 
 
-def step2_min_size(variant=-1):
-    """Smallest (nx, ny) the two-step kernel takes with tile shape ``variant``."""
+def step_multi_info(variant=-1):
+    """(steps per launch, smallest nx, smallest ny) of multi-step variant ``variant``."""
     import ctypes
-    a, b = ctypes.c_int64(), ctypes.c_int64()
-    nat.call("lbm_step2_min_size", int(variant), ctypes.byref(a), ctypes.byref(b))
-    return a.value, b.value
+    k, a, b = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    nat.call("lbm_step_multi_info", int(variant), ctypes.byref(k), ctypes.byref(a),
+             ctypes.byref(b))
+    return k.value, a.value, b.value
 
 
-def step2(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
-          layout=nat.LAYOUT_SOA):
-    """Two fused steps ``src -> dst`` in one launch (temporal blocking),
-    bitwise equal to two ``step`` calls; ``moments`` receive the
-    pre-collision moments of the second step."""
-    nx, ny = _populations(src, "step2", layout)
+def step_multi(src, dst, omega, bc=nat.BC_CAVITY, u0=0.1, moments=None, variant=-1,
+               layout=nat.LAYOUT_SOA):
+    """K fused steps ``src -> dst`` in one launch (temporal blocking; K from
+    ``step_multi_info(variant)``), bitwise equal to K ``step`` calls;
+    ``moments`` receive the pre-collision moments of the last step."""
+    nx, ny = _populations(src, "step_multi", layout)
     _require_device(dst)
-    suf = _suffix(src, "step2")
-    _same(src, dst, "step2")
+    suf = _suffix(src, "step_multi")
+    _same(src, dst, "step_multi")
     if dst.shape != src.shape:
-        raise ValueError("step2: output must match the input")
+        raise ValueError("step_multi: output must match the input")
     rho = ux = uy = None
     if moments is not None:
         rho, ux, uy = moments
         _require_device(rho, ux, uy)
         for t in (rho, ux, uy):
-            _same(src, t, "step2")
+            _same(src, t, "step_multi")
             if t.shape != (nx, ny):
-                raise ValueError("step2: moment arrays must have shape (nx, ny)")
-    nat.call("lbm_step2_" + suf, nat.ptr(src), nat.ptr(dst), nx, ny, int(layout), float(omega),
-             int(bc), float(u0), nat.ptr(rho), nat.ptr(ux), nat.ptr(uy), int(variant),
-             nat.stream_ptr())
+                raise ValueError("step_multi: moment arrays must have shape (nx, ny)")
+    nat.call("lbm_step_multi_" + suf, nat.ptr(src), nat.ptr(dst), nx, ny, int(layout),
+             float(omega), int(bc), float(u0), nat.ptr(rho), nat.ptr(ux), nat.ptr(uy),
+             int(variant), nat.stream_ptr())
     return dst
 
 

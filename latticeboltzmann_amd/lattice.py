@@ -45,12 +45,16 @@ class Lattice:
     bc : ``"cavity"`` (walls + moving lid at y = ny-1) or ``"periodic"``.
     u0 : lid velocity (cavity only).
     dtype : float64 or float32 storage and arithmetic.
-    variant : launch shape of the fused step (-1 = library default).
+    variant : launch shape of the fused step (-1 = library default, "auto" =
+        autotuned; "auto" also tunes the two-step kernel's tile shape).
     layout : ``"rows"`` (default) or ``"soa"`` (the reference's f[i][k][l]).
+    multi_step : advance runs of K steps with the temporally blocked
+        multi-step kernel (``lbm_step_multi_*``, K = 2..4 per launch, bitwise
+        equal to K single steps) where the grid is large enough for its tiles.
     """
 
     def __init__(self, nx, ny, omega, bc="cavity", u0=0.1, dtype=torch.float64, device=None,
-                 variant=-1, layout="rows"):
+                 variant=-1, layout="rows", multi_step=True):
         nat.require_gpu()
         if bc not in BCS:
             raise ValueError("bc must be 'cavity' or 'periodic'")
@@ -61,6 +65,14 @@ class Lattice:
         if layout not in ("rows", "soa"):
             raise ValueError("layout must be 'rows' or 'soa'")
         self.layout = nat.LAYOUT_ROWS if layout == "rows" else nat.LAYOUT_SOA
+        self.multi_variant, self.multi_k = None, 1
+        if multi_step:
+            if variant == "auto":
+                from .autotune import multi_variant
+                self.set_multi_variant(multi_variant(self.nx, self.ny, self.dtype, self.layout,
+                                                     BCS[bc]))
+            else:
+                self.set_multi_variant(-1)
         if variant == "auto":
             from .autotune import step_variant
             variant = step_variant(self.nx, self.ny, self.dtype, self.layout, BCS[bc])
@@ -115,12 +127,38 @@ class Lattice:
         return self.populations_device().cpu().numpy()
 
     # -- stepping ------------------------------------------------------------
+    def set_multi_variant(self, variant):
+        """Use multi-step variant ``variant`` (None: single steps only); a
+        variant whose tiles do not fit the grid is ignored."""
+        self.multi_variant, self.multi_k = None, 1
+        if variant is None:
+            return
+        k, mnx, mny = kernels.step_multi_info(variant)
+        if self.nx >= mnx and self.ny >= mny:
+            self.multi_variant, self.multi_k = int(variant), k
+
+    def _runs(self, n):
+        return n // self.multi_k if self.multi_variant is not None else 0
+
+    def launches(self, n):
+        """Kernel launches (buffer swaps) that ``step(n)`` makes."""
+        q = self._runs(n)
+        return q + (n - q * self.multi_k)
+
     def step(self, n=1, moments=None):
         """Advance ``n`` steps.  ``moments=(rho, ux, uy)`` tensors receive the
-        pre-collision moments of the last step."""
+        pre-collision moments of the last step.  Runs of K steps go through the
+        multi-step kernel when enabled; the bits are those of n single steps."""
         bc = BCS[self.bc]
-        for s in range(n):
-            mom = moments if (moments is not None and s == n - 1) else None
+        runs = self._runs(n)
+        single = n - runs * self.multi_k
+        for p in range(runs):
+            mom = moments if (moments is not None and p == runs - 1 and single == 0) else None
+            kernels.step_multi(self.f, self._g, self.omega, bc, self.u0, moments=mom,
+                               variant=self.multi_variant, layout=self.layout)
+            self.f, self._g = self._g, self.f
+        for s in range(single):
+            mom = moments if (moments is not None and s == single - 1) else None
             kernels.step(self.f, self._g, self.omega, bc, self.u0, moments=mom,
                          variant=self.variant, layout=self.layout)
             self.f, self._g = self._g, self.f
@@ -145,8 +183,8 @@ class Lattice:
         """Step until max|u(t + every) - u(t)| / u_ref < tol (FP64 only).
 
         ``u_ref`` defaults to the lid velocity.  With ``graph`` (and an even
-        ``every``) the ``every`` steps between two residual checks are
-        captured once into a HIP graph and replayed, which removes the
+        number of launches per ``every`` steps) the steps between two residual
+        checks are captured once into a HIP graph and replayed, which removes the
         per-launch overhead that dominates small grids.  Returns the number of
         steps taken and the last residual.
         """
@@ -155,7 +193,7 @@ class Lattice:
         uyp = torch.zeros_like(uxp)
         kernels.velocity_residual(self.f, uxp, uyp, layout=self.layout)
         taken, res = 0, float("inf")
-        with StepGraph(self, every, enabled=graph and every % 2 == 0) as advance:
+        with StepGraph(self, every, enabled=graph) as advance:
             while taken < max_steps:
                 advance()
                 taken += every
@@ -172,8 +210,9 @@ class Lattice:
 class StepGraph:
     """Context manager returning a callable that advances ``lat`` by ``n`` steps.
 
-    Enabled, the n launches (n even, so the state ends in the same buffer)
-    are captured once into a HIP graph on a side stream (capture needs a
+    Enabled, the launches of n steps (an even number of them, so the state
+    ends in the buffer it started in; otherwise the steps run eagerly) are
+    captured once into a HIP graph on a side stream (capture needs a
     stream of its own) and each call replays it on the caller's current
     stream, in order with the work around it; disabled, each call launches
     the steps eagerly.  Replaying on the capture stream instead needs two
@@ -182,7 +221,9 @@ class StepGraph:
     """
 
     def __init__(self, lat, n, enabled=True):
-        self.lat, self.n, self.enabled = lat, int(n), bool(enabled) and int(n) % 2 == 0
+        self.lat, self.n = lat, int(n)
+        swaps = lat.launches(self.n) if hasattr(lat, "launches") else self.n
+        self.enabled = bool(enabled) and swaps % 2 == 0
         self.graph = None
 
     def __enter__(self):

@@ -38,6 +38,17 @@ DEFAULT_SCALE = {"f32": 1.0, "bf16": 1.0, "f16": 1024.0}
 BYTES = {"f32": 4, "bf16": 2, "f16": 2}
 ROUNDING = {"nearest": nat.ROUND_NEAREST, "stochastic": nat.ROUND_STOCHASTIC}
 COMPUTE = {"f32": nat.COMPUTE_F32, "f64": nat.COMPUTE_F64}
+SEED_STRIDE = 0x85EBCA77  # dither seed increment from one step to the next
+
+
+def multi_info(variant=-1):
+    """(steps per launch, smallest nx, smallest ny) of deviation multi-step
+    variant ``variant``."""
+    import ctypes
+    k, a, b = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
+    nat.call("lbm_dev_multi_info", int(variant), ctypes.byref(k), ctypes.byref(a),
+             ctypes.byref(b))
+    return k.value, a.value, b.value
 
 
 class DeviationLattice:
@@ -47,7 +58,8 @@ class DeviationLattice:
     """
 
     def __init__(self, nx, ny, omega, bc="cavity", u0=0.1, fmt="f16", scale=None, device=None,
-                 variant=-1, layout="rows", rounding="stochastic", seed=0, compute="f32"):
+                 variant=-1, layout="rows", rounding="stochastic", seed=0, compute="f32",
+                 multi_step="auto"):
         nat.require_gpu()
         if fmt not in FORMATS:
             raise ValueError("fmt must be one of %s" % sorted(FORMATS))
@@ -70,11 +82,56 @@ class DeviationLattice:
         shape = (self.nx, 9, self.ny) if self.layout == nat.LAYOUT_ROWS else (9, self.nx, self.ny)
         self.d = torch.empty(shape, dtype=_RAW[self.fmt], device=self.device)
         self._e = torch.empty_like(self.d)
+        self.multi_variant, self.multi_k = None, 1
         if variant == "auto":
             variant = self._tune()
+            if multi_step == "auto":
+                self._tune_multi(int(variant))
         self.variant = int(variant)
+        if multi_step is True:
+            self.set_multi_variant(-1)
+        elif not isinstance(multi_step, (bool, str)) and multi_step is not None:
+            self.set_multi_variant(int(multi_step))
         self.time = 0
         self.init_equilibrium()
+
+    def set_multi_variant(self, variant):
+        """Advance runs of K steps with multi-step variant ``variant`` of
+        ``lbm_dev_step_multi`` (None: single steps); a variant whose tiles do
+        not fit the grid is ignored."""
+        self.multi_variant, self.multi_k = None, 1
+        if variant is None:
+            return
+        k, mnx, mny = multi_info(variant)
+        if self.nx >= mnx and self.ny >= mny:
+            self.multi_variant, self.multi_k = int(variant), k
+
+    def _tune_multi(self, single):
+        """Pick single steps or a multi-step shape, whichever is faster per step."""
+        from .autotune import best_variant
+        n_var = nat.load().lbm_dev_multi_variant_count()
+        info = {v: multi_info(v) for v in range(n_var)}
+        cands = ["s%d" % single] + ["m%d" % v for v, (k, a, b) in info.items()
+                                    if self.nx >= a and self.ny >= b]
+
+        def make_launch(nx, ny):
+            a = torch.zeros_like(self.d)
+            b = torch.empty_like(self.d)
+
+            def launch(c):
+                if c[0] == "s":
+                    self._call_step(a, b, int(c[1:]))
+                else:
+                    self._call_multi(a, b, int(c[1:]))
+            return launch
+
+        def per_step(times):
+            return {c: t / (1 if c[0] == "s" else info[int(c[1:])][0]) for c, t in times.items()}
+
+        best = best_variant("dev_multi%d_c%d_r%d" % (n_var, self.compute, self.rounding), self.nx,
+                            self.ny, self.fmt_name, self.layout, self._bc, make_launch, cands,
+                            normalize=per_step)
+        self.set_multi_variant(int(best[1:]) if best[0] == "m" else None)
 
     def _tune(self):
         from .autotune import best_variant
@@ -124,16 +181,44 @@ class DeviationLattice:
         return self.populations_device().cpu().numpy()
 
     # -- stepping ----------------------------------------------------------------
-    def _call_step(self, src, dst, variant, ux=None, uy=None, step_index=0):
+    def _seed(self, step_index):
         # a different dither stream every step, reproducible from (seed, time)
-        seed = (self.seed * 0x9E3779B1 + step_index * 0x85EBCA77 + 1) & 0xFFFFFFFF
+        return (self.seed * 0x9E3779B1 + step_index * SEED_STRIDE + 1) & 0xFFFFFFFF
+
+    def _call_step(self, src, dst, variant, ux=None, uy=None, step_index=0):
         nat.call("lbm_dev_step", nat.ptr(src), nat.ptr(dst), self.fmt, self.nx, self.ny,
                  self.layout, self.omega, self._bc, self.u0, self.scale, self.compute,
-                 self.rounding, seed, nat.ptr(ux), nat.ptr(uy), int(variant), nat.stream_ptr())
+                 self.rounding, self._seed(step_index), nat.ptr(ux), nat.ptr(uy), int(variant),
+                 nat.stream_ptr())
 
-    def step(self, n=1):
-        for _ in range(n):
-            self._call_step(self.d, self._e, self.variant, step_index=self.time)
+    def _call_multi(self, src, dst, variant, ux=None, uy=None, step_index=0):
+        nat.call("lbm_dev_step_multi", nat.ptr(src), nat.ptr(dst), self.fmt, self.nx, self.ny,
+                 self.layout, self.omega, self._bc, self.u0, self.scale, self.compute,
+                 self.rounding, self._seed(step_index), SEED_STRIDE, nat.ptr(ux), nat.ptr(uy),
+                 int(variant), nat.stream_ptr())
+
+    def launches(self, n):
+        """Kernel launches (buffer swaps) that ``step(n)`` makes."""
+        q = n // self.multi_k if self.multi_variant is not None else 0
+        return q + (n - q * self.multi_k)
+
+    def step(self, n=1, velocity=None):
+        """Advance ``n`` steps (runs of K through the multi-step kernel when
+        enabled: the same bits as single steps).  ``velocity=(ux, uy)`` FP32
+        tensors receive the pre-collision velocity of the last step."""
+        runs = n // self.multi_k if self.multi_variant is not None else 0
+        single = n - runs * self.multi_k
+        ux = uy = None
+        for r in range(runs):
+            if velocity is not None and r == runs - 1 and single == 0:
+                ux, uy = velocity
+            self._call_multi(self.d, self._e, self.multi_variant, ux, uy, step_index=self.time)
+            self.d, self._e = self._e, self.d
+            self.time += self.multi_k
+        for s_ in range(single):
+            if velocity is not None and s_ == single - 1:
+                ux, uy = velocity
+            self._call_step(self.d, self._e, self.variant, ux, uy, step_index=self.time)
             self.d, self._e = self._e, self.d
             self.time += 1
         return self

@@ -1,0 +1,112 @@
+"""The multi-step (temporally blocked) kernel against single steps, bit for bit."""
+
+import numpy as np
+import pytest
+import torch
+
+from latticeboltzmann_amd import _native as nat
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(777)
+
+
+def populations(nx, ny, dtype):
+    from oracle import d2q9_numpy as o
+    f = o.weights(np.float64)[:, None, None] * (1 + 0.2 * RNG.standard_normal((9, nx, ny)))
+    return torch.from_numpy(f).to(dtype).cuda()
+
+
+def to_layout(soa, layout):
+    from latticeboltzmann_amd import kernels
+    return soa.clone() if layout == nat.LAYOUT_SOA else kernels.relayout(soa, nat.LAYOUT_SOA, layout)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("bc", [nat.BC_CAVITY, nat.BC_PERIODIC])
+@pytest.mark.parametrize("layout", [nat.LAYOUT_ROWS, nat.LAYOUT_SOA])
+def test_multi_step_equals_single_steps(gpu, dtype, bc, layout):
+    from latticeboltzmann_amd import kernels
+    nvar = nat.load().lbm_step_multi_variant_count()
+    for v in range(nvar):
+        K, mnx, mny = kernels.step_multi_info(v)
+        for nx, ny in [(mnx, mny), (mnx + 27, mny + 131), (130, 1030)]:
+            print("multi", dtype, bc, layout, v, K, nx, ny, flush=True)
+            f = to_layout(populations(nx, ny, dtype), layout)
+            a = f.clone()
+            b = torch.empty_like(f)
+            m1 = [torch.empty((nx, ny), dtype=dtype, device="cuda") for _ in range(3)]
+            m2 = [torch.empty((nx, ny), dtype=dtype, device="cuda") for _ in range(3)]
+            for s in range(K):
+                kernels.step(a, b, 1.7, bc, 0.1, moments=m1 if s == K - 1 else None, layout=layout)
+                a, b = b, a
+            c = torch.empty_like(f)
+            kernels.step_multi(f, c, 1.7, bc, 0.1, moments=m2, variant=v, layout=layout)
+            assert torch.equal(a, c), (v, K, nx, ny)
+            for x, y in zip(m1, m2):
+                assert torch.equal(x, y)
+
+
+def test_multi_step_rejects_small_grids(gpu):
+    from latticeboltzmann_amd import kernels
+    K, mnx, mny = kernels.step_multi_info(0)
+    f = populations(mnx - 1, mny, torch.float64)
+    with pytest.raises(ValueError):
+        kernels.step_multi(f, torch.empty_like(f), 1.0, variant=0)
+    with pytest.raises(ValueError):
+        kernels.step_multi(f, f, 1.0, variant=0)
+
+
+def test_lattice_multi_step_run(gpu):
+    # a Lattice advancing 2..4 steps per launch reproduces single steps, with
+    # moments and odd step counts
+    from latticeboltzmann_amd.lattice import Lattice
+    nvar = nat.load().lbm_step_multi_variant_count()
+    ref = Lattice(96, 200, 1.3, "cavity", 0.1, multi_step=False)
+    mr = [torch.empty((96, 200), dtype=torch.float64, device="cuda") for _ in range(3)]
+    ref.step(37, moments=mr)
+    for v in range(nvar):
+        lat = Lattice(96, 200, 1.3, "cavity", 0.1)
+        lat.set_multi_variant(v)
+        m = [torch.empty_like(x) for x in mr]
+        lat.step(37, moments=m)
+        assert lat.launches(37) < 37
+        assert torch.equal(lat.f, ref.f), v
+        for x, y in zip(m, mr):
+            assert torch.equal(x, y)
+
+
+@pytest.mark.parametrize("fmt", ["f16", "bf16", "f32"])
+@pytest.mark.parametrize("compute", ["f32", "f64"])
+@pytest.mark.parametrize("rounding", ["stochastic", "nearest"])
+def test_dev_multi_step_equals_single_steps(gpu, fmt, compute, rounding):
+    # stored deviations: each intermediate step of a multi-step launch rounds
+    # (and dithers) exactly as a single-step launch does
+    from latticeboltzmann_amd.precision import DeviationLattice, multi_info
+    nvar = nat.load().lbm_dev_multi_variant_count()
+    for bc in ("cavity", "periodic"):
+        for v in range(nvar):
+            K, mnx, mny = multi_info(v)
+            nx, ny = mnx + 5, mny + 37
+            print("dev multi", fmt, compute, rounding, bc, v, K, nx, ny, flush=True)
+            rho = 1 + 0.01 * RNG.standard_normal((nx, ny))
+            ux, uy = 0.05 * RNG.standard_normal((nx, ny)), 0.05 * RNG.standard_normal((nx, ny))
+            kw = dict(fmt=fmt, rounding=rounding, compute=compute, seed=11)
+            ref = DeviationLattice(nx, ny, 1.3, bc, 0.1, multi_step=False, **kw)
+            lat = DeviationLattice(nx, ny, 1.3, bc, 0.1, multi_step=v, **kw)
+            assert lat.multi_k == K
+            ref.init_equilibrium(rho, ux, uy)
+            lat.init_equilibrium(rho, ux, uy)
+            vr = [torch.empty((nx, ny), dtype=torch.float32, device="cuda") for _ in range(2)]
+            vl = [torch.empty_like(t) for t in vr]
+            ref.step(2 * K + 1, velocity=vr)
+            lat.step(2 * K + 1, velocity=vl)
+            assert lat.launches(2 * K + 1) == 3
+            assert torch.equal(ref.d, lat.d), (bc, v)
+            for a, b in zip(vr, vl):
+                assert torch.equal(a, b)
+            lat.step(K, velocity=vl)
+            ref.step(K, velocity=vr)
+            assert torch.equal(ref.d, lat.d)
+            for a, b in zip(vr, vl):
+                assert torch.equal(a, b)
