@@ -103,6 +103,7 @@ int lbm_step_multi_f32(const float *src, float *dst, int64_t nx, int64_t ny, int
                        float omega, int bc, float u0, float *rho, float *ux, float *uy,
                        int variant, void *stream);
 int lbm_step_multi_variant_count(void);
+int lbm_step_multi_default_variant(void); /* the variant that -1 selects */
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 
 /* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with
@@ -219,6 +220,7 @@ int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t 
                        uint32_t seed, uint32_t seed_stride, float *ux, float *uy, int variant,
                        void *stream);
 int lbm_dev_multi_variant_count(void);
+int lbm_dev_multi_default_variant(void); /* the variant that -1 selects */
 int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 /* *out = sum_{i,k,l} f = nx*ny + (sum of stored values) / scale, in FP64     */
 int lbm_dev_mass(const void *d, int64_t nx, int64_t ny, int fmt, double scale, double *work,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "lbm_step_multi_f32": (_i32, [_vp, _vp, _i64, _i64, _i32, _f32, _i32, _f32, _vp, _vp, _vp,
                                   _i32, _vp]),
     "lbm_step_multi_variant_count": (_i32, []),
+    "lbm_step_multi_default_variant": (_i32, []),
     "lbm_step_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
                                    ctypes.POINTER(_i64)]),
     "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),
@@ -97,6 +98,7 @@ SIGNATURES = {
                                   _i32, _i32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _i32,
                                   _vp]),
     "lbm_dev_multi_variant_count": (_i32, []),
+    "lbm_dev_multi_default_variant": (_i32, []),
     "lbm_dev_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
                                   ctypes.POINTER(_i64)]),
     "lbm_dev_mass": (_i32, [_vp, _i64, _i64, _i32, _f64, _vp, _vp, _vp]),

@@ -663,6 +663,8 @@ int lbm_step_variant_count(void) { return kNumStepShapes; }
 
 int lbm_step_multi_variant_count(void) { return kNumMultiShapes; }
 
+int lbm_step_multi_default_variant(void) { return kDefaultMultiShape; }
+
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
     if (!steps || !min_nx || !min_ny) return LBM_E_NULL;
     return multi_shape_info(variant, steps, min_nx, min_ny);

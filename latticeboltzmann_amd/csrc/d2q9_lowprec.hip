@@ -539,6 +539,8 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
 
 int lbm_dev_multi_variant_count(void) { return kNumDevMultiShapes; }
 
+int lbm_dev_multi_default_variant(void) { return kDefaultDevMultiShape; }
+
 int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
     if (!steps || !min_nx || !min_ny) return LBM_E_NULL;
     const int shape = variant < 0 ? kDefaultDevMultiShape : variant;

@@ -133,6 +133,8 @@ class Lattice:
         self.multi_variant, self.multi_k = None, 1
         if variant is None:
             return
+        if variant < 0:
+            variant = nat.load().lbm_step_multi_default_variant()
         k, mnx, mny = kernels.step_multi_info(variant)
         if self.nx >= mnx and self.ny >= mny:
             self.multi_variant, self.multi_k = int(variant), k

@@ -102,6 +102,8 @@ class DeviationLattice:
         self.multi_variant, self.multi_k = None, 1
         if variant is None:
             return
+        if variant < 0:
+            variant = nat.load().lbm_dev_multi_default_variant()
         k, mnx, mny = multi_info(variant)
         if self.nx >= mnx and self.ny >= mny:
             self.multi_variant, self.multi_k = int(variant), k
