@@ -110,3 +110,26 @@ def test_dev_multi_step_equals_single_steps(gpu, fmt, compute, rounding):
             assert torch.equal(ref.d, lat.d)
             for a, b in zip(vr, vl):
                 assert torch.equal(a, b)
+
+
+@pytest.mark.parametrize("bc", [nat.BC_CAVITY, nat.BC_PERIODIC])
+def test_multi_step_large_ragged_grid(gpu, bc):
+    # 4099 x 8191 FP64: no side is a multiple of a tile, 2.4 GB per buffer
+    from latticeboltzmann_amd import kernels
+    nx, ny = 4099, 8191
+    print("multi large", bc, nx, ny, flush=True)
+    rows = 1 + 0.01 * torch.rand((nx, ny), dtype=torch.float64, device="cuda")
+    ux = 0.05 * torch.rand((nx, ny), dtype=torch.float64, device="cuda")
+    uy = 0.05 * torch.rand((nx, ny), dtype=torch.float64, device="cuda")
+    f = kernels.relayout(kernels.equilibrium(rows, ux, uy), nat.LAYOUT_SOA, nat.LAYOUT_ROWS)
+    del rows, ux, uy
+    for v in (nat.load().lbm_step_multi_default_variant(), 0, 12):
+        K = kernels.step_multi_info(v)[0]
+        a, b = f.clone(), torch.empty_like(f)
+        for _ in range(K):
+            kernels.step(a, b, 0.9, bc, 0.1, layout=nat.LAYOUT_ROWS)
+            a, b = b, a
+        kernels.step_multi(f, b, 0.9, bc, 0.1, variant=v, layout=nat.LAYOUT_ROWS)
+        assert torch.equal(a, b), v
+        del a, b
+        torch.cuda.empty_cache()
