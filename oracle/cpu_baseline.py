@@ -3,11 +3,15 @@
 TEST / MEASUREMENT INFRASTRUCTURE ONLY (see oracle/__init__.py).
 
 Runs in its own process pinned to one core, with BLAS/OpenMP threads set to
-one, so the figure is a single-core rate.  Two legs:
+one, so the figure is a single-core rate.  Three legs:
 
 * ``numpy``: the numpy opt1 restatement (cavity_opt1.py:100-245), the
   reference's optimised pure-numpy flavour;
-* ``c``: the single-thread C restatement of the same step (-O2).
+* ``opt2``: the structure of cavity_opt2.py:272-277, numpy
+  stream_and_bounce_back followed by the C restatement of _lbkernels.collide
+  (d2q9_colliden, c/d2q9.h:121-131; single thread, -O2, as the reference's
+  setup.py builds it);
+* ``c``: the single-thread C restatement of the fused step (-O2).
 
 Prints one JSON object: MLUPS per leg, steps timed, grid, core count used.
 """
@@ -89,11 +93,20 @@ def main():
     c_step(state)
     steps_c, t_c = time_leg(c_step, state, args.budget, args.max_steps)
 
+    def opt2_step(g2):
+        d2q9_numpy.stream_and_bounce_back(g2, 0.1)
+        ctypes_ref.lbk_collide(g2.reshape(9, -1), args.omega)
+
+    f2 = d2q9_numpy.cavity_init(n, n)
+    opt2_step(f2)
+    steps_o2, t_o2 = time_leg(opt2_step, f2, args.budget, args.max_steps)
+
     print(json.dumps({
         "grid": [n, n],
         "numpy_opt1": {"steps": steps_np, "seconds": t_np,
                        "mlups": n * n * steps_np / t_np / 1e6},
         "c_port": {"steps": steps_c, "seconds": t_c, "mlups": n * n * steps_c / t_c / 1e6},
+        "opt2": {"steps": steps_o2, "seconds": t_o2, "mlups": n * n * steps_o2 / t_o2 / 1e6},
         "cores": cores,
         "cpu_count": os.cpu_count(),
         "cpu_model": cpu_model(),
