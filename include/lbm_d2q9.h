@@ -210,15 +210,19 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
                  float omega, int bc, float u0, float scale, int compute, int rounding,
                  uint32_t seed, float *ux, float *uy, int variant, void *stream);
 /* K steps src -> dst in one launch (temporal blocking, K = 2..4 fixed by the
- * variant), as lbm_step_multi_* for stored deviations: LDS holds the stored
- * values, so step j of the launch (j = 0..K-1) rounds exactly as a
- * lbm_dev_step call with seed + j * seed_stride (mod 2^32), and the result is
- * bitwise that of those K calls.  ux/uy receive the last step's velocity.
- * lbm_dev_multi_info gives the variant's K and the smallest grid it takes.    */
+ * variant), as lbm_step_multi_* for stored deviations.  round_each = 1: LDS
+ * holds the stored values, so step j of the launch (j = 0..K-1) rounds exactly
+ * as a lbm_dev_step call with seed + j * seed_stride (mod 2^32), and the
+ * result is bitwise that of those K calls.  round_each = 0 (store every K
+ * steps): the intermediate steps stay unrounded in the compute type and only
+ * the last is rounded, with seed + (K-1) * seed_stride; fewer roundings, so a
+ * different and smaller rounding error (LBM_E_ARG for an FP64-compute shape
+ * whose tile would not fit the 160 KB of LDS).  ux/uy receive the last step's
+ * velocity.  lbm_dev_multi_info gives the variant's K and smallest grid.      */
 int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
                        float omega, int bc, float u0, float scale, int compute, int rounding,
-                       uint32_t seed, uint32_t seed_stride, float *ux, float *uy, int variant,
-                       void *stream);
+                       uint32_t seed, uint32_t seed_stride, int round_each, float *ux, float *uy,
+                       int variant, void *stream);
 int lbm_dev_multi_variant_count(void);
 int lbm_dev_multi_default_variant(void); /* the variant that -1 selects */
 int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);

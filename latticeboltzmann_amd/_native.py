@@ -95,8 +95,8 @@ SIGNATURES = {
     "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32, _i32,
                             _i32, ctypes.c_uint32, _vp, _vp, _i32, _vp]),
     "lbm_dev_step_multi": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32,
-                                  _i32, _i32, ctypes.c_uint32, ctypes.c_uint32, _vp, _vp, _i32,
-                                  _vp]),
+                                  _i32, _i32, ctypes.c_uint32, ctypes.c_uint32, _i32, _vp, _vp,
+                                  _i32, _vp]),
     "lbm_dev_multi_variant_count": (_i32, []),
     "lbm_dev_multi_default_variant": (_i32, []),
     "lbm_dev_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),

@@ -178,6 +178,37 @@ __device__ __forceinline__ void cavity_walls_dev(CT g[9], const typename Codec<F
     }
 }
 
+// The same walls on values already in the compute type (c[i * cs] is the
+// node's own pre-stream value of channel i): the multi-step kernel's phases
+// that keep unrounded values between steps.
+template <typename CT>
+__device__ __forceinline__ void cavity_walls_ct(CT g[9], const CT *c, int64_t cs, int k, int l,
+                                                int nx, int ny, CT u0) {
+    if (l == 0) {
+        g[2] = c[4 * cs];
+        g[5] = c[7 * cs];
+        g[6] = c[8 * cs];
+    }
+    if (l == ny - 1) {
+        const CT c2 = c[2 * cs], c5 = c[5 * cs], c6 = c[6 * cs];
+        const CT rl = CT(1) + (((c6 + c2) + (c5 + g[6])) + ((g[2] + g[5]) + (g[3] + (g[0] + g[1]))));
+        const CT t = (CT(6) / CT(36)) * rl * u0;
+        g[4] = c2;
+        g[8] = c6 + t;
+        g[7] = c5 - t;
+    }
+    if (k == 0) {
+        g[1] = c[3 * cs];
+        g[5] = c[7 * cs];
+        g[8] = c[6 * cs];
+    }
+    if (k == nx - 1) {
+        g[3] = c[1 * cs];
+        g[6] = c[8 * cs];
+        g[7] = c[5 * cs];
+    }
+}
+
 // BGK collide on deviations in the compute type CT (float or double).
 template <typename CT>
 __device__ __forceinline__ void collide_dev(CT d[9], CT omega, CT &ux, CT &uy) {

@@ -175,6 +175,128 @@ __device__ __forceinline__ void dev_phase_to_global(const typename Codec<FMT>::r
     }
 }
 
+// Phases that keep the unrounded compute-type values between steps (the
+// store-every-K form: populations reach the storage format once per launch).
+template <int FMT, typename CT, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::raw *__restrict__ src,
+                                                    CT *out, int k0, int l0, int nx, int ny,
+                                                    Layout L, CT omega, CT u0, CT inv_scale) {
+    using C = Codec<FMT>;
+    using R = typename C::raw;
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+        const PullRows<R> rows(src, L, k, nx);
+        R raw[9];
+        pull9<false>(rows, l, ny, raw);
+        CT g[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
+                                      inv_scale);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+    }
+}
+
+template <typename CT, int BC, int TR, int TC, int H, int BS>
+__device__ __forceinline__ void dev_phase_lds_ct(const CT *in, CT *out, int k0, int l0, int nx,
+                                                 int ny, CT omega, CT u0) {
+    constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
+    constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
+    for (int n = threadIdx.x; n < N; n += BS) {
+        const int r = n / W, c = n - r * W;
+        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+        const int m = (r + 1) * Wi + (c + 1);
+        CT g[9];
+        pull_lds(in, Ni, Wi, m, g);
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls_ct<CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+    }
+}
+
+template <int FMT, typename CT, int BC, bool MOM, int TR, int TC, int BS>
+__device__ __forceinline__ void dev_phase_to_global_ct(const CT *in,
+                                                       typename Codec<FMT>::raw *__restrict__ dst,
+                                                       int k0, int l0, int nx, int ny, Layout L,
+                                                       CT omega, CT u0, CT scale, int stochastic,
+                                                       uint32_t seed, float *__restrict__ ux_o,
+                                                       float *__restrict__ uy_o) {
+    using R = typename Codec<FMT>::raw;
+    constexpr int Wi = TC + 2, Ni = (TR + 2) * Wi;
+    for (int n = threadIdx.x; n < TR * TC; n += BS) {
+        const int r = n / TC, c = n - r * TC;
+        const int k = k0 + r, l = l0 + c;
+        if (k >= nx || l >= ny) continue;
+        const int m = (r + 1) * Wi + (c + 1);
+        CT g[9];
+        pull_lds(in, Ni, Wi, m, g);
+        if constexpr (BC == LBM_BC_CAVITY) cavity_walls_ct<CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+        R raw[9];
+        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+        R *d = dst + int64_t(k) * L.rs + l;
+#pragma unroll
+        for (int i = 0; i < 9; ++i) d[i * L.cs] = raw[i];
+        if constexpr (MOM) {
+            const int64_t j = int64_t(k) * ny + l;
+            ux_o[j] = float(ux);
+            uy_o[j] = float(uy);
+        }
+    }
+}
+
+// Store-every-K form: the K - 1 intermediate steps stay unrounded in LDS (in
+// the compute type) and only the last is rounded to the storage format, with
+// the dither of the launch's last step (seed + (K-1) * seed_stride).  Fewer
+// roundings than K single steps, hence different (smaller) rounding error.
+template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
+__global__ __launch_bounds__(BS) void k_dev_stepk_once(
+    const typename Codec<FMT>::raw *__restrict__ src, typename Codec<FMT>::raw *__restrict__ dst,
+    int nx, int ny, Layout L, int ntc, int ntiles, CT omega, CT u0, CT scale, CT inv_scale,
+    int stochastic, uint32_t seed, uint32_t seed_stride, float *__restrict__ ux_o,
+    float *__restrict__ uy_o) {
+    static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
+    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
+    __shared__ CT sa[9 * NA];
+    __shared__ CT sb[9 * NB];
+    const int tile = xcd_tile(ntiles);
+    if (tile >= ntiles) return;  // whole block: before any barrier
+    const int tr = tile / ntc, tc = tile - tr * ntc;
+    const int k0 = tr * TR, l0 = tc * TC;
+    const uint32_t last = seed + uint32_t(K - 1) * seed_stride;
+
+    dev_phase_global_ct<FMT, CT, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0,
+                                                        inv_scale);
+    __syncthreads();
+    if constexpr (K == 2) {
+        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
+                                                             scale, stochastic, last, ux_o, uy_o);
+    } else if constexpr (K == 3) {
+        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0,
+                                                             scale, stochastic, last, ux_o, uy_o);
+    } else {
+        dev_phase_lds_ct<CT, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
+                                                             scale, stochastic, last, ux_o, uy_o);
+    }
+}
+
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
 __global__ __launch_bounds__(BS) void k_dev_stepk(const typename Codec<FMT>::raw *__restrict__ src,
                                                   typename Codec<FMT>::raw *__restrict__ dst,
@@ -396,25 +518,43 @@ constexpr int kDefaultDevMultiShape = 2;
 
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t seed_stride,
-                       hipStream_t s) {
+                       bool round_each, hipStream_t s) {
     using R = typename Codec<FMT>::raw;
     if (a.nx < TR + 2 * K || a.ny < TC + 2 * K) return LBM_E_SIZE;
     const int64_t ntr = (a.nx + TR - 1) / TR, ntc = (a.ny + TC - 1) / TC;
     const int64_t ntiles = ntr * ntc;
     const int64_t nblocks = ((ntiles + 7) / 8) * 8;
     if (nblocks > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL((k_dev_stepk<FMT, CT, BC, MOM, K, TR, TC, BS>), dim3(unsigned(nblocks)),
-                       dim3(BS), 0, s, static_cast<const R *>(src), static_cast<R *>(dst), a.nx,
-                       a.ny, a.L, int(ntc), int(ntiles), CT(a.omega), CT(a.u0), CT(a.scale),
-                       CT(1) / CT(a.scale), a.stochastic, a.seed, seed_stride, a.ux, a.uy);
+    if (round_each)
+        hipLaunchKernelGGL((k_dev_stepk<FMT, CT, BC, MOM, K, TR, TC, BS>), dim3(unsigned(nblocks)),
+                           dim3(BS), 0, s, static_cast<const R *>(src), static_cast<R *>(dst),
+                           a.nx, a.ny, a.L, int(ntc), int(ntiles), CT(a.omega), CT(a.u0),
+                           CT(a.scale), CT(1) / CT(a.scale), a.stochastic, a.seed, seed_stride,
+                           a.ux, a.uy);
+    else {
+        // unrounded values in the compute type: an FP64 tile can outgrow the
+        // 160 KB of LDS of a gfx950 CU, and such shapes are refused
+        constexpr int64_t NA = int64_t(TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
+        constexpr int64_t NB = K >= 3 ? int64_t(TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
+        constexpr int64_t lds = 9 * (NA + NB) * int64_t(sizeof(CT));
+        if constexpr (lds <= 160 * 1024) {
+            hipLaunchKernelGGL((k_dev_stepk_once<FMT, CT, BC, MOM, K, TR, TC, BS>),
+                               dim3(unsigned(nblocks)), dim3(BS), 0, s, static_cast<const R *>(src),
+                               static_cast<R *>(dst), a.nx, a.ny, a.L, int(ntc), int(ntiles),
+                               CT(a.omega), CT(a.u0), CT(a.scale), CT(1) / CT(a.scale),
+                               a.stochastic, a.seed, seed_stride, a.ux, a.uy);
+        } else {
+            return LBM_E_ARG;
+        }
+    }
     return LBM_OK;
 }
 
 template <int FMT, typename CT, int BC, bool MOM>
-int launch_dev_multi_bc(const void *src, void *dst, const DevArgs &a, uint32_t stride, int shape,
-                        hipStream_t s) {
+int launch_dev_multi_bc(const void *src, void *dst, const DevArgs &a, uint32_t stride, bool each,
+                        int shape, hipStream_t s) {
 #define LBM_DEV_LAUNCH_MULTI(id, k, tr, tc, bs) \
-    case id: return launch_dev_multi_v<FMT, CT, BC, MOM, k, tr, tc, bs>(src, dst, a, stride, s);
+    case id: return launch_dev_multi_v<FMT, CT, BC, MOM, k, tr, tc, bs>(src, dst, a, stride, each, s);
     switch (shape) {
         LBM_DEV_MULTI_SHAPES(LBM_DEV_LAUNCH_MULTI)
         default: return LBM_E_ARG;
@@ -424,28 +564,28 @@ int launch_dev_multi_bc(const void *src, void *dst, const DevArgs &a, uint32_t s
 
 template <int FMT, typename CT>
 int launch_dev_multi_fmt(const void *src, void *dst, int bc, const DevArgs &a, uint32_t stride,
-                         int shape, hipStream_t s) {
+                         bool each, int shape, hipStream_t s) {
     const bool mom = a.ux != nullptr;
     if (bc == LBM_BC_CAVITY)
-        return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, true>(src, dst, a, stride, shape, s)
-                   : launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, false>(src, dst, a, stride, shape, s);
-    return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, true>(src, dst, a, stride, shape, s)
-               : launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, false>(src, dst, a, stride, shape, s);
+        return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, true>(src, dst, a, stride, each, shape, s)
+                   : launch_dev_multi_bc<FMT, CT, LBM_BC_CAVITY, false>(src, dst, a, stride, each, shape, s);
+    return mom ? launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, true>(src, dst, a, stride, each, shape, s)
+               : launch_dev_multi_bc<FMT, CT, LBM_BC_PERIODIC, false>(src, dst, a, stride, each, shape, s);
 }
 
 int launch_dev_multi(const void *src, void *dst, int fmt, int compute, int bc, const DevArgs &a,
-                     uint32_t stride, int shape, hipStream_t s) {
+                     uint32_t stride, bool each, int shape, hipStream_t s) {
     if (compute == LBM_COMPUTE_F64) {
         switch (fmt) {
-            case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, double>(src, dst, bc, a, stride, shape, s);
-            case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, double>(src, dst, bc, a, stride, shape, s);
-            default: return launch_dev_multi_fmt<LBM_FMT_F16, double>(src, dst, bc, a, stride, shape, s);
+            case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, double>(src, dst, bc, a, stride, each, shape, s);
+            case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, double>(src, dst, bc, a, stride, each, shape, s);
+            default: return launch_dev_multi_fmt<LBM_FMT_F16, double>(src, dst, bc, a, stride, each, shape, s);
         }
     }
     switch (fmt) {
-        case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, float>(src, dst, bc, a, stride, shape, s);
-        case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, float>(src, dst, bc, a, stride, shape, s);
-        default: return launch_dev_multi_fmt<LBM_FMT_F16, float>(src, dst, bc, a, stride, shape, s);
+        case LBM_FMT_F32: return launch_dev_multi_fmt<LBM_FMT_F32, float>(src, dst, bc, a, stride, each, shape, s);
+        case LBM_FMT_BF16: return launch_dev_multi_fmt<LBM_FMT_BF16, float>(src, dst, bc, a, stride, each, shape, s);
+        default: return launch_dev_multi_fmt<LBM_FMT_F16, float>(src, dst, bc, a, stride, each, shape, s);
     }
 }
 
@@ -555,8 +695,8 @@ int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny
 
 int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
                        float omega, int bc, float u0, float scale, int compute, int rounding,
-                       uint32_t seed, uint32_t seed_stride, float *ux, float *uy, int variant,
-                       void *stream) {
+                       uint32_t seed, uint32_t seed_stride, int round_each, float *ux, float *uy,
+                       int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if ((ux == nullptr) != (uy == nullptr)) return LBM_E_NULL;
     if (!valid_grid(nx, ny) || nx > INT32_MAX || ny > INT32_MAX) return LBM_E_SIZE;
@@ -570,8 +710,9 @@ int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t 
     const int shape = variant < 0 ? kDefaultDevMultiShape : variant;
     const DevArgs a{int(nx), int(ny), make_layout(layout, nx, ny), omega, u0, scale,
                     rounding == LBM_ROUND_STOCHASTIC ? 1 : 0, seed, ux, uy};
-    const int rc = launch_dev_multi(src, dst, fmt, compute, bc, a, seed_stride, shape,
-                                    as_stream(stream));
+    if (round_each != 0 && round_each != 1) return LBM_E_ARG;
+    const int rc = launch_dev_multi(src, dst, fmt, compute, bc, a, seed_stride, round_each != 0,
+                                    shape, as_stream(stream));
     return rc ? rc : launch_status();
 }
 

@@ -59,7 +59,7 @@ class DeviationLattice:
 
     def __init__(self, nx, ny, omega, bc="cavity", u0=0.1, fmt="f16", scale=None, device=None,
                  variant=-1, layout="rows", rounding="stochastic", seed=0, compute="f32",
-                 multi_step="auto"):
+                 multi_step="auto", store_every=1):
         nat.require_gpu()
         if fmt not in FORMATS:
             raise ValueError("fmt must be one of %s" % sorted(FORMATS))
@@ -82,24 +82,67 @@ class DeviationLattice:
         shape = (self.nx, 9, self.ny) if self.layout == nat.LAYOUT_ROWS else (9, self.nx, self.ny)
         self.d = torch.empty(shape, dtype=_RAW[self.fmt], device=self.device)
         self._e = torch.empty_like(self.d)
-        self.multi_variant, self.multi_k = None, 1
-        if variant == "auto":
+        self.multi_variant, self.multi_k, self.round_each = None, 1, True
+        self.store_every = int(store_every)
+        if self.store_every not in (1, 2, 3, 4):
+            raise ValueError("store_every must be 1, 2, 3 or 4")
+        auto = variant == "auto"
+        if auto:
             variant = self._tune()
-            if multi_step == "auto":
-                self._tune_multi(int(variant))
         self.variant = int(variant)
-        if multi_step is True:
+        if self.store_every > 1:
+            self._set_store_every(self.store_every, tune=multi_step == "auto" and auto)
+        elif multi_step == "auto" and auto:
+            self._tune_multi(self.variant)
+        elif multi_step is True:
             self.set_multi_variant(-1)
         elif not isinstance(multi_step, (bool, str)) and multi_step is not None:
             self.set_multi_variant(int(multi_step))
         self.time = 0
         self.init_equilibrium()
 
+    # shapes tried first for each K when storing every K steps
+    _STORE_EVERY_SHAPES = {2: 1, 3: 2, 4: 4}
+
+    def _set_store_every(self, K, tune):
+        """Store every K steps: runs of K steps through one multi-step launch
+        whose K - 1 intermediate steps stay unrounded on chip."""
+        n_var = nat.load().lbm_dev_multi_variant_count()
+        fits = [v for v in range(n_var) if multi_info(v)[0] == K
+                and self.nx >= multi_info(v)[1] and self.ny >= multi_info(v)[2]]
+        if not fits:
+            raise ValueError("the grid is too small for %d steps per launch" % K)
+        # FP64-compute tiles that do not fit LDS are refused by the library
+        ok = []
+        a = torch.zeros_like(self.d)
+        b = torch.empty_like(self.d)
+        for v in fits:
+            try:
+                self._call_multi(a, b, v, round_each=False)
+                ok.append(v)
+            except ValueError:
+                pass
+        if not ok:
+            raise ValueError("no tile shape with %d steps per launch fits this format" % K)
+        best = self._STORE_EVERY_SHAPES[K] if self._STORE_EVERY_SHAPES[K] in ok else ok[0]
+        if tune and len(ok) > 1:
+            from .autotune import best_variant
+
+            def make_launch(nx, ny):
+                def launch(v):
+                    self._call_multi(a, b, v, round_each=False)
+                return launch
+
+            best = best_variant("dev_once%d_k%d_c%d_r%d" % (n_var, K, self.compute, self.rounding),
+                                self.nx, self.ny, self.fmt_name, self.layout, self._bc,
+                                make_launch, ok)
+        self.multi_variant, self.multi_k, self.round_each = int(best), K, False
+
     def set_multi_variant(self, variant):
         """Advance runs of K steps with multi-step variant ``variant`` of
-        ``lbm_dev_step_multi`` (None: single steps); a variant whose tiles do
-        not fit the grid is ignored."""
-        self.multi_variant, self.multi_k = None, 1
+        ``lbm_dev_step_multi`` (None: single steps), rounding every step; a
+        variant whose tiles do not fit the grid is ignored."""
+        self.multi_variant, self.multi_k, self.round_each = None, 1, True
         if variant is None:
             return
         if variant < 0:
@@ -193,11 +236,11 @@ class DeviationLattice:
                  self.rounding, self._seed(step_index), nat.ptr(ux), nat.ptr(uy), int(variant),
                  nat.stream_ptr())
 
-    def _call_multi(self, src, dst, variant, ux=None, uy=None, step_index=0):
+    def _call_multi(self, src, dst, variant, ux=None, uy=None, step_index=0, round_each=True):
         nat.call("lbm_dev_step_multi", nat.ptr(src), nat.ptr(dst), self.fmt, self.nx, self.ny,
                  self.layout, self.omega, self._bc, self.u0, self.scale, self.compute,
-                 self.rounding, self._seed(step_index), SEED_STRIDE, nat.ptr(ux), nat.ptr(uy),
-                 int(variant), nat.stream_ptr())
+                 self.rounding, self._seed(step_index), SEED_STRIDE, int(bool(round_each)),
+                 nat.ptr(ux), nat.ptr(uy), int(variant), nat.stream_ptr())
 
     def launches(self, n):
         """Kernel launches (buffer swaps) that ``step(n)`` makes."""
@@ -206,15 +249,18 @@ class DeviationLattice:
 
     def step(self, n=1, velocity=None):
         """Advance ``n`` steps (runs of K through the multi-step kernel when
-        enabled: the same bits as single steps).  ``velocity=(ux, uy)`` FP32
-        tensors receive the pre-collision velocity of the last step."""
+        enabled: the same bits as single steps, or, with ``store_every`` K,
+        rounded to the storage format once per K steps; a remainder of fewer
+        than K steps runs as single steps).  ``velocity=(ux, uy)`` FP32 tensors
+        receive the pre-collision velocity of the last step."""
         runs = n // self.multi_k if self.multi_variant is not None else 0
         single = n - runs * self.multi_k
         ux = uy = None
         for r in range(runs):
             if velocity is not None and r == runs - 1 and single == 0:
                 ux, uy = velocity
-            self._call_multi(self.d, self._e, self.multi_variant, ux, uy, step_index=self.time)
+            self._call_multi(self.d, self._e, self.multi_variant, ux, uy, step_index=self.time,
+                             round_each=self.round_each)
             self.d, self._e = self._e, self.d
             self.time += self.multi_k
         for s_ in range(single):

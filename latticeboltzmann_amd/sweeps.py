@@ -9,7 +9,8 @@
   with the FP32 collide and with an FP64 collide (FP32 storage always uses
   the FP64 collide here: an FP32 collide stores unrounded FP32 results and
   shares the bias of the 16-bit FP32 collide).
-* The FP16 cavity is also run with the FP64 collide.
+* The FP16 cavity is also run with the FP64 collide, and FP16/BF16 storing
+  every 3 steps (intermediate steps unrounded on chip).
 
 Usage::
 
@@ -41,7 +42,8 @@ def _timed(fn):
     return out, time.perf_counter() - t
 
 
-def cavity_precision(n=8192, re=1000.0, u=0.1, steps=50000, formats=FORMATS, log=None):
+def cavity_precision(n=8192, re=1000.0, u=0.1, steps=50000, formats=FORMATS, log=None,
+                     store_every=(3,)):
     om = omega_for_cavity(n, re, u)
     ref = Lattice(n, n, om, "cavity", u, torch.float64, variant="auto")
     m0 = ref.mass()
@@ -54,15 +56,20 @@ def cavity_precision(n=8192, re=1000.0, u=0.1, steps=50000, formats=FORMATS, log
     torch.cuda.empty_cache()
     if log:
         log(json.dumps({"cavity": "f64", **out["f64"]}))
-    for fmt, compute in [(f, "f32") for f in formats] + [("f16", "f64")]:
-        lat = DeviationLattice(n, n, om, "cavity", u, fmt=fmt, variant="auto", compute=compute)
+    runs = [(f, "f32", 1) for f in formats] + [("f16", "f64", 1)]
+    runs += [(f, "f32", k) for k in store_every for f in ("bf16", "f16")]
+    for fmt, compute, every in runs:
+        lat = DeviationLattice(n, n, om, "cavity", u, fmt=fmt, variant="auto", compute=compute,
+                               store_every=every)
         _, sec = _timed(lambda: lat.step(steps))
         _, ux, uy = lat.moments()
         rec = velocity_errors(ux, uy, uxr, uyr, u)
         m = lat.mass()
         rec.update(seconds=sec, mlups=n * n * steps / sec / 1e6, mass_drift=m / m0 - 1,
-                   mass_vs_f64=m / m64 - 1, compute=compute)
+                   mass_vs_f64=m / m64 - 1, compute=compute, store_every=every)
         key = fmt if compute == "f32" else fmt + "_f64_collide"
+        if every > 1:
+            key += "_store_every_%d" % every
         out[key] = rec
         if log:
             log(json.dumps({"cavity": key, **rec}))

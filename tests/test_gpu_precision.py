@@ -201,3 +201,39 @@ def test_shear_wave_viscosity_fp16(gpu):
         lat.step(200)
     nu = o.fit_viscosity(ts, amps, n)
     assert abs(nu / o.viscosity_theory(omega) - 1) < 1e-2
+
+
+@pytest.mark.parametrize("fmt", lp.FORMATS)
+@pytest.mark.parametrize("compute", ["f32", "f64"])
+@pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
+@pytest.mark.parametrize("K", [2, 3, 4])
+def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
+    # storing every K steps: K unrounded steps on chip, then one compensated
+    # rounding, so the stored state lies within the one-step rounding bounds of
+    # the FP64 oracle run for K steps (plus K steps of FP32 arithmetic drift)
+    from latticeboltzmann_amd.precision import DeviationLattice
+    w = o.weights()[:, None, None]
+    s = SCALES[fmt]
+    om, u0 = float(np.float32(1.3)), float(np.float32(0.1))
+    shape = (24, 80)
+    f = populations(*shape)
+    for bc in ("cavity", "periodic"):
+        print("store every", fmt, compute, rounding, K, bc, flush=True)
+        lat = DeviationLattice(*shape, om, bc, u0, fmt=fmt, scale=s, rounding=rounding,
+                               compute=compute, store_every=K)
+        assert lat.multi_k == K and not lat.round_each
+        lat.load(f)
+        stored = lat.d.cpu().numpy().view(np.float32 if fmt == "f32" else np.uint16)
+        expect = lp.decode(stored, fmt, s, "rows")
+        for _ in range(K):
+            (o.cavity_step if bc == "cavity" else o.periodic_step)(expect, *((om, u0) if bc == "cavity" else (om,)))
+        lat.step(K)
+        x_exp, x_got = (expect - w) * s, (lat.populations() - w) * s
+        drift = (3 * K if compute == "f32" else 0) * lp.ulp(np.abs(x_exp).max(0), "f32")[None] + 1e-8 * s
+        if fmt == "f32" and compute == "f32":
+            tol = drift + lp.ulp(x_exp, "f32")
+        else:
+            big = lp.ulp(np.abs(x_exp).max(0), fmt)
+            tol = (4 if rounding == "nearest" else 6) * big[None] + drift
+        err = np.abs(x_got - x_exp)
+        assert (err <= tol).all(), (bc, float((err / tol).max()))
