@@ -15,7 +15,7 @@
 Usage::
 
     python -m latticeboltzmann_amd.sweeps [--cavity-n 8192] [--shear-n 4096] [--steps 50000] \
-        [--out sweep.json]
+        [--no-shear] [--out sweep.json]
 """
 
 import argparse
@@ -130,11 +130,13 @@ def main(argv=None):
     ap.add_argument("--shear-n", type=int, default=4096)
     ap.add_argument("--steps", type=int, default=50000)
     ap.add_argument("--shear-every", type=int, default=2500)
+    ap.add_argument("--no-shear", action="store_true", help="run the cavity sweep only")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
-    res = {"cavity": cavity_precision(a.cavity_n, steps=a.steps, log=log),
-           "shear": shear_precision(a.shear_n, steps=a.steps, every=a.shear_every, log=log)}
+    res = {"cavity": cavity_precision(a.cavity_n, steps=a.steps, log=log)}
+    if not a.no_shear:
+        res["shear"] = shear_precision(a.shear_n, steps=a.steps, every=a.shear_every, log=log)
     if a.out:
         with open(a.out, "w") as fh:
             json.dump(res, fh, indent=1)
