@@ -52,3 +52,29 @@ def test_size_class_rounds_up_to_powers_of_two():
     assert size_class(1000, 1000) == "1024x1024"
     assert size_class(64, 65) == "64x128"
     assert size_class(1, 1) == "1x1"
+
+
+def _bare(cls, **attrs):
+    # the host arithmetic of a lattice without touching a device
+    obj = cls.__new__(cls)
+    obj.__dict__.update(attrs)
+    return obj
+
+
+def test_lattice_launch_count():
+    from latticeboltzmann_amd.lattice import Lattice, StepGraph
+    lat = _bare(Lattice, multi_variant=10, multi_k=3)
+    assert [lat.launches(n) for n in (0, 1, 2, 3, 4, 1000, 1001)] == [0, 1, 2, 1, 2, 334, 335]
+    single = _bare(Lattice, multi_variant=None, multi_k=1)
+    assert single.launches(1000) == 1000
+    # a graph is captured only when the state ends in the buffer it started in
+    assert StepGraph(lat, 1000).enabled and not StepGraph(lat, 999).enabled
+    assert not StepGraph(_bare(Lattice, multi_variant=12, multi_k=4), 1001).enabled
+    assert StepGraph(single, 2).enabled and not StepGraph(single, 3).enabled
+
+
+def test_deviation_lattice_launch_count():
+    from latticeboltzmann_amd.precision import DeviationLattice
+    lat = _bare(DeviationLattice, multi_variant=2, multi_k=3)
+    assert [lat.launches(n) for n in (1, 3, 7, 9)] == [1, 1, 3, 3]
+    assert _bare(DeviationLattice, multi_variant=None, multi_k=1).launches(7) == 7
