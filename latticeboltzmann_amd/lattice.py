@@ -46,7 +46,7 @@ class Lattice:
     u0 : lid velocity (cavity only).
     dtype : float64 or float32 storage and arithmetic.
     variant : launch shape of the fused step (-1 = library default, "auto" =
-        autotuned; "auto" also tunes the two-step kernel's tile shape).
+        autotuned; "auto" also tunes the multi-step kernel's shape).
     layout : ``"rows"`` (default) or ``"soa"`` (the reference's f[i][k][l]).
     multi_step : advance runs of K steps with the temporally blocked
         multi-step kernel (``lbm_step_multi_*``, K = 2..4 per launch, bitwise

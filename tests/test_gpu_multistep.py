@@ -55,6 +55,21 @@ def test_multi_step_rejects_small_grids(gpu):
         kernels.step_multi(f, torch.empty_like(f), 1.0, variant=0)
     with pytest.raises(ValueError):
         kernels.step_multi(f, f, 1.0, variant=0)
+    g = populations(64, 128, torch.float64)
+    h = torch.empty_like(g)
+    lib, sp = nat.load(), nat.stream_ptr()
+    args = lambda **kw: dict(dict(layout=nat.LAYOUT_SOA, bc=nat.BC_CAVITY, variant=-1), **kw)  # noqa: E731
+    for kw, code in ((args(layout=7), -3), (args(bc=5), -3), (args(variant=99), -3)):
+        assert lib.lbm_step_multi_f64(g.data_ptr(), h.data_ptr(), 64, 128, kw["layout"], 1.0, kw["bc"],
+                                      0.1, None, None, None, kw["variant"], sp) == code
+    # moments all or none
+    m = torch.empty((64, 128), dtype=torch.float64, device="cuda")
+    assert lib.lbm_step_multi_f64(g.data_ptr(), h.data_ptr(), 64, 128, nat.LAYOUT_SOA, 1.0,
+                                  nat.BC_CAVITY, 0.1, m.data_ptr(), None, None, -1, sp) == -1
+    k = __import__("ctypes").c_int()
+    a, b = __import__("ctypes").c_int64(), __import__("ctypes").c_int64()
+    assert lib.lbm_step_multi_info(99, k, a, b) == -3
+    torch.cuda.synchronize()
 
 
 def test_lattice_multi_step_run(gpu):
