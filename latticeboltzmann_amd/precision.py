@@ -114,8 +114,8 @@ class DeviationLattice:
             raise ValueError("the grid is too small for %d steps per launch" % K)
         # FP64-compute tiles that do not fit LDS are refused by the library
         ok = []
-        a = torch.zeros_like(self.d)
-        b = torch.empty_like(self.d)
+        # the state buffers serve as scratch: the state is initialised after tuning
+        a, b = self.d.zero_(), self._e
         for v in fits:
             try:
                 self._call_multi(a, b, v, round_each=False)
@@ -160,8 +160,7 @@ class DeviationLattice:
                                     if self.nx >= a and self.ny >= b]
 
         def make_launch(nx, ny):
-            a = torch.zeros_like(self.d)
-            b = torch.empty_like(self.d)
+            a, b = self.d.zero_(), self._e  # scratch: the state is initialised after tuning
 
             def launch(c):
                 if c[0] == "s":
@@ -182,8 +181,7 @@ class DeviationLattice:
         from .autotune import best_variant
 
         def make_launch(nx, ny):
-            a = torch.zeros_like(self.d)
-            b = torch.empty_like(self.d)
+            a, b = self.d.zero_(), self._e  # scratch: the state is initialised after tuning
 
             def launch(v):
                 self._call_step(a, b, v)
