@@ -13,9 +13,11 @@ gives the viscosity, against the theory nu = (1/omega - 1/2)/3.
   middle of the box, (2/n) sum u sin (``shear_wave_opt1.py:174``), or as the
   reference's ``shear_wave_decay.py:155-158`` does, the largest entry of
   (2/n) |FFT(u)| of that line.
-* Fits in FP64: least squares of log A against t, or the one-parameter form
+* Fits in FP64: least squares of log A against t, the one-parameter form
   of ``shear_wave_decay.py:72-73, 167`` (A0 fixed, least squares in A) by
-  Gauss-Newton.
+  Gauss-Newton, or the two-point log ratio of ``:526``.
+* ``omega_viscosity_sweep``: the omega -> nu experiment of ``:479-538``, every
+  omega in one batched launch per step.
 
 The lattice is the FP64 (or FP32) ``Lattice``; the amplitude is read from the
 device moments (one line per sample).
@@ -74,6 +76,40 @@ def fit_exponential(t, amplitude, wavelength, a0, nu0=None, iters=50):
         if abs(step) <= 1e-15 * max(abs(nu), 1e-300):
             break
     return nu
+
+
+def viscosity_log_ratio(a0, a_t, t, wavelength):
+    """nu = -ln(A(t) / A0) / (k^2 t), k = 2 pi / wavelength: the two-point
+    estimate of ``shear_wave_decay.py:526``."""
+    k = 2 * math.pi / wavelength
+    return -math.log(a_t / a0) / (k * k * t)
+
+
+def omega_viscosity_sweep(omegas, n=320, steps=1000, amplitude=0.1, fmt="f32"):
+    """The omega -> nu experiment of ``shear_wave_decay.py:479-538``: one n x n
+    shear wave (u_x = A sin(2 pi y / n)) per omega, ``steps`` steps, nu from
+    the log ratio of the final to the initial amplitude (projection on the
+    initial sine).  With n a multiple of 64 every omega runs in one batched
+    lattice set (one persistent launch per step, ``fmt`` storage, FP64
+    collide); otherwise one FP64 lattice per omega.  Returns (nu, nu_theory)
+    as arrays in the order of ``omegas``."""
+    omegas = [float(w) for w in omegas]
+    if n % 64 == 0 and steps % 2 == 0:
+        from .batch import BatchedLattices
+        inst = [dict(nx=n, ny=n, omega=w, amplitude=amplitude, direction="x", bc="periodic")
+                for w in omegas]
+        bat = BatchedLattices(inst, fmt=fmt)
+        times, amps = bat.run(steps, sample_every=steps)
+        nu = [viscosity_log_ratio(amps[0, i], amps[-1, i], times[-1] - times[0], n)
+              for i in range(len(omegas))]
+    else:
+        nu = []
+        for w in omegas:
+            sw = ShearWave(n, n, w, amplitude, "x")
+            a0 = sw.amplitude()
+            sw.lat.step(steps)
+            nu.append(viscosity_log_ratio(a0, sw.amplitude(), steps, n))
+    return np.asarray(nu), viscosity_theory(np.asarray(omegas))
 
 
 class ShearWave:

@@ -26,3 +26,20 @@ def test_shear_wave_matches_oracle_and_theory(gpu, direction):
     print(direction, nu_lin / nu_theory - 1, nu_exp / nu_theory - 1)
     assert abs(nu_lin / nu_theory - 1) < 1e-3
     assert abs(nu_exp / nu_theory - 1) < 1e-2
+
+
+def test_omega_viscosity_sweep_batched(gpu):
+    # shear_wave_decay.py:479-538: one launch per step for every omega; the
+    # two-point estimate against the same estimate on the FP64 oracle
+    import math
+    from latticeboltzmann_amd.shear import omega_viscosity_sweep
+    omegas = [0.3, 0.65, 1.0, 1.35, 1.7]
+    nu, theory = omega_viscosity_sweep(omegas, n=64, steps=1000)
+    for w, got, th in zip(omegas, nu, theory):
+        f = o.shear_wave_init(64, 64, 0.1, "x")
+        a0 = o.shear_wave_amplitude(f, "x")
+        for _ in range(1000):
+            o.periodic_step(f, w)
+        ref = -math.log(o.shear_wave_amplitude(f, "x") / a0) / ((2 * math.pi / 64) ** 2 * 1000)
+        assert abs(got / ref - 1) < 1e-3, (w, got, ref)
+        assert abs(got / th - 1) < 0.04, (w, got, th)  # the initial layer of the f_eq start

@@ -38,3 +38,12 @@ def test_fits_recover_nu_from_exact_decay():
 def test_viscosity_theory():
     assert viscosity_theory(1.0) == 1.0 / 6.0
     assert abs(viscosity_theory(1.7) - (1 / 1.7 - 0.5) / 3) < 1e-16
+
+
+def test_viscosity_log_ratio_exact():
+    import math
+    from latticeboltzmann_amd.shear import viscosity_log_ratio
+    nu, n, t = 0.0731, 300, 1000
+    k2 = (2 * math.pi / n) ** 2
+    a_t = 0.1 * math.exp(-nu * k2 * t)
+    assert abs(viscosity_log_ratio(0.1, a_t, t, n) - nu) < 1e-14
