@@ -185,10 +185,12 @@ def cpu_baseline(n, budget):
         "kind": "port",
         "sample": "opt2 structure of the %dx%d FP64 cavity step (numpy stream_and_bounce_back + C "
                   "restatement of _lbkernels.collide, -O2), %d steps (%.1f s) on one core of %s "
-                  "(%d visible); numpy opt1 restatement: %.3f MLUPS; fused C restatement -O2: "
-                  "%.3f MLUPS"
+                  "(%d visible); the same with the collide built -O3 -march=native: %s MLUPS; "
+                  "numpy opt1 restatement: %.3f MLUPS; fused C restatement -O2: %.3f MLUPS"
                   % (n, n, rec["opt2"]["steps"], rec["opt2"]["seconds"], rec["cpu_model"],
-                     rec["cpu_count"], rec["numpy_opt1"]["mlups"], rec["c_port"]["mlups"]),
+                     rec["cpu_count"],
+                     "%.3f" % rec["opt2_native"]["mlups"] if rec.get("opt2_native") else "n/a",
+                     rec["numpy_opt1"]["mlups"], rec["c_port"]["mlups"]),
         "numpy_opt1_mlups": round(rec["numpy_opt1"]["mlups"], 4),
     }
 

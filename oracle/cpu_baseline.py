@@ -11,7 +11,10 @@ one, so the figure is a single-core rate.  Three legs:
   stream_and_bounce_back followed by the C restatement of _lbkernels.collide
   (d2q9_colliden, c/d2q9.h:121-131; single thread, -O2, as the reference's
   setup.py builds it);
-* ``c``: the single-thread C restatement of the fused step (-O2).
+* ``c``: the single-thread C restatement of the fused step (-O2);
+* ``opt2_native``: the opt2 leg with the collide compiled -O3 -march=native
+  on the host that runs this (the variant the measurement plan reports
+  beside -O2), built into a temporary directory; skipped without a compiler.
 
 Prints one JSON object: MLUPS per leg, steps timed, grid, core count used.
 """
@@ -44,6 +47,27 @@ def cpu_model():
     except OSError:
         pass
     return platform.processor() or "unknown"
+
+
+def native_collide():
+    """oracle_lbk_collide_f64 from d2q9_ref.c built -O3 -march=native for this
+    host (in a temporary directory), or None when no compiler is available."""
+    import ctypes
+    import shutil
+    import subprocess
+    import tempfile
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if not cc:
+        return None
+    out = os.path.join(tempfile.mkdtemp(prefix="lbm_oracle_native_"), "liboracle_native.so")
+    src = os.path.join(os.path.dirname(os.path.abspath(__file__)), "d2q9_ref.c")
+    r = subprocess.run([cc, "-O3", "-march=native", "-ffp-contract=off", "-fPIC", "-shared",
+                        "-std=c11", "-o", out, src], capture_output=True)
+    if r.returncode:
+        return None
+    fn = ctypes.CDLL(out).oracle_lbk_collide_f64
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_double]
+    return fn
 
 
 def time_leg(fn, f, budget_s, max_steps):
@@ -101,12 +125,25 @@ def main():
     opt2_step(f2)
     steps_o2, t_o2 = time_leg(opt2_step, f2, args.budget, args.max_steps)
 
+    native = native_collide()
+    native_rec = None
+    if native is not None:
+        def opt2_native_step(g2):
+            d2q9_numpy.stream_and_bounce_back(g2, 0.1)
+            native(ctypes_ref._ptr(g2), g2[0].size, args.omega)
+
+        f3 = d2q9_numpy.cavity_init(n, n)
+        opt2_native_step(f3)
+        steps_n, t_n = time_leg(opt2_native_step, f3, args.budget, args.max_steps)
+        native_rec = {"steps": steps_n, "seconds": t_n, "mlups": n * n * steps_n / t_n / 1e6}
+
     print(json.dumps({
         "grid": [n, n],
         "numpy_opt1": {"steps": steps_np, "seconds": t_np,
                        "mlups": n * n * steps_np / t_np / 1e6},
         "c_port": {"steps": steps_c, "seconds": t_c, "mlups": n * n * steps_c / t_c / 1e6},
         "opt2": {"steps": steps_o2, "seconds": t_o2, "mlups": n * n * steps_o2 / t_o2 / 1e6},
+        "opt2_native": native_rec,
         "cores": cores,
         "cpu_count": os.cpu_count(),
         "cpu_model": cpu_model(),
