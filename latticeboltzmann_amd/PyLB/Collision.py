@@ -32,7 +32,7 @@ import numbers
 import numpy as np
 import torch
 
-from .. import kernels
+from latticeboltzmann_amd import kernels
 from . import _staging
 
 _FLOAT_TYPES = (np.dtype(np.float32), np.dtype(np.float64))

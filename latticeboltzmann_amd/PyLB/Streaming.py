@@ -19,7 +19,7 @@ tensors on a ROCm device are processed on the device.
 import numpy as np
 import torch
 
-from .. import kernels
+from latticeboltzmann_amd import kernels
 from . import _staging
 
 # Channel velocities, as PyLB/Streaming.py:28-29 exposes them.

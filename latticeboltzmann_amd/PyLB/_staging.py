@@ -8,7 +8,7 @@ tensors already on a ROCm device are used as they are.
 import numpy as np
 import torch
 
-from .. import _native as nat
+from latticeboltzmann_amd import _native as nat
 
 _TORCH_OF = {np.dtype(np.float64): torch.float64, np.dtype(np.float32): torch.float32}
 

@@ -85,3 +85,26 @@ def test_dropin_rejects_like_the_reference_bindings():
         PyLB.stream(np.zeros((8, 3, 3)))
     with pytest.raises(TypeError):
         PyLB.stream_and_bounce_back(np.zeros((9, 3, 3), dtype=np.int32))
+
+
+def test_dropin_imports_as_top_level_pylb():
+    """The reference's env.sh:17 puts its package directory on PYTHONPATH and
+    the drivers say `import PyLB` (cavity_opt2.py:43, tests/02-CollideTest.py:27).
+    The same import, with this package's directory on the path, gives the
+    GPU functions and the IO module."""
+    import subprocess
+    import sys
+    env = dict(os.environ)
+    env["PYTHONPATH"] = os.pathsep.join(
+        [os.path.join(ROOT, "latticeboltzmann_amd"), ROOT, env.get("PYTHONPATH", "")])
+    code = ("import PyLB, PyLB.IO as IO\n"
+            "from latticeboltzmann_amd import kernels\n"
+            "assert PyLB.__file__.endswith('latticeboltzmann_amd/PyLB/__init__.py'), PyLB.__file__\n"
+            "names = ['equilibrium', 'collide', 'stream', 'stream_and_bounce_back']\n"
+            "assert all(callable(getattr(PyLB, n)) for n in names)\n"
+            "assert callable(IO.save_mpiio)\n"
+            "print('ok')\n")
+    out = subprocess.run([sys.executable, "-c", code], env=env, cwd="/tmp",
+                         capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert out.stdout.strip() == "ok"
