@@ -120,24 +120,44 @@ __global__ __launch_bounds__(BS) void k_step(const T *__restrict__ src, T *__res
 // Requires nx >= TR + 2K, ny >= TC + 2K.
 
 // One phase from global memory: step of the nodes of a halo-H tile ring.
+// When the ring has more nodes than the block has threads, a thread takes two
+// nodes per pass and issues both nodes' pulls before either collide, so the
+// threads that own a second node do not hold the block at the next barrier
+// for a second, serial round of HBM latency.
 template <typename T, int BC, int TR, int TC, int H, int BS>
 __device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *out, int k0, int l0,
                                                   int nx, int ny, Layout L, T omega, T u0) {
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
-        const int r = n / W, c = n - r * W;
-        int k = k0 - H + r, l = l0 - H + c;
-        k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
-        l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
-        const PullRows<T> rows(src, L, k, nx);
-        T g[9];
-        pull9<false>(rows, l, ny, g);
-        if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0);
-        T rho, ux, uy;
-        collide_opt1(g, omega, rho, ux, uy);
+    constexpr int V = N > BS ? 2 : 1;
+    for (int n0 = threadIdx.x; n0 < N; n0 += V * BS) {
+        T g[V][9];
+        int kv[V], lv[V];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            const int r = n / W, c = n - r * W;
+            int k = k0 - H + r, l = l0 - H + c;
+            k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
+            l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
+            kv[v] = k;
+            lv[v] = l;
+            if (n < N) {
+                const PullRows<T> rows(src, L, k, nx);
+                pull9<false>(rows, l, ny, g[v]);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            if (n >= N) continue;
+            const int k = kv[v], l = lv[v];
+            if constexpr (BC == LBM_BC_CAVITY)
+                cavity_walls(g[v], src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0);
+            T rho, ux, uy;
+            collide_opt1(g[v], omega, rho, ux, uy);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) out[i * N + n] = g[v][i];
+        }
     }
 }
 
