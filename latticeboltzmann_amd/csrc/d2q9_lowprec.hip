@@ -87,24 +87,40 @@ __device__ __forceinline__ void dev_phase_global(const typename Codec<FMT>::raw 
     using C = Codec<FMT>;
     using R = typename C::raw;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
-        const int r = n / W, c = n - r * W;
-        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
-        const PullRows<R> rows(src, L, k, nx);
-        R raw[9];
-        pull9<false>(rows, l, ny, raw);
-        CT g[9];
+    constexpr int V = N > BS ? 2 : 1;  // both pulls of a thread before either collide
+    for (int n0 = threadIdx.x; n0 < N; n0 += V * BS) {
+        R raw[V][9];
+        int kv[V], lv[V];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
-        if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
-                                      inv_scale);
-        CT ux, uy;
-        collide_dev<CT>(g, omega, ux, uy);
-        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
-        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            const int r = n / W, c = n - r * W;
+            kv[v] = wrap1(k0 - H + r, nx);
+            lv[v] = wrap1(l0 - H + c, ny);
+            if (n < N) {
+                const PullRows<R> rows(src, L, kv[v], nx);
+                pull9<false>(rows, lv[v], ny, raw[v]);
+            }
+        }
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = raw[i];
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            if (n >= N) continue;
+            const int k = kv[v], l = lv[v];
+            CT g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[v][i])) * inv_scale;
+            if constexpr (BC == LBM_BC_CAVITY)
+                cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
+                                          inv_scale);
+            CT ux, uy;
+            collide_dev<CT>(g, omega, ux, uy);
+            Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
+            R q[9];
+            store_values<FMT, CT>(g, scale, stochastic, dither, q);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) out[i * N + n] = q[i];
+        }
     }
 }
 
@@ -184,22 +200,37 @@ __device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::r
     using C = Codec<FMT>;
     using R = typename C::raw;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
-        const int r = n / W, c = n - r * W;
-        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
-        const PullRows<R> rows(src, L, k, nx);
-        R raw[9];
-        pull9<false>(rows, l, ny, raw);
-        CT g[9];
+    constexpr int V = N > BS ? 2 : 1;  // both pulls of a thread before either collide
+    for (int n0 = threadIdx.x; n0 < N; n0 += V * BS) {
+        R raw[V][9];
+        int kv[V], lv[V];
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
-        if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
-                                      inv_scale);
-        CT ux, uy;
-        collide_dev<CT>(g, omega, ux, uy);
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            const int r = n / W, c = n - r * W;
+            kv[v] = wrap1(k0 - H + r, nx);
+            lv[v] = wrap1(l0 - H + c, ny);
+            if (n < N) {
+                const PullRows<R> rows(src, L, kv[v], nx);
+                pull9<false>(rows, lv[v], ny, raw[v]);
+            }
+        }
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+        for (int v = 0; v < V; ++v) {
+            const int n = n0 + v * BS;
+            if (n >= N) continue;
+            const int k = kv[v], l = lv[v];
+            CT g[9];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[v][i])) * inv_scale;
+            if constexpr (BC == LBM_BC_CAVITY)
+                cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
+                                          inv_scale);
+            CT ux, uy;
+            collide_dev<CT>(g, omega, ux, uy);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+        }
     }
 }
 
