@@ -110,7 +110,7 @@ __global__ __launch_bounds__(BS) void k_step(const T *__restrict__ src, T *__res
 // and a halo ring K-1 nodes deep, at wrapped global coordinates, with the
 // single-step node code (pull from src, walls, collide), into LDS; each later
 // phase computes the next step on a ring one node shallower from the previous
-// phase's LDS values (two LDS buffers, used in turn); the last phase writes the
+// phase's LDS values, in place in the one LDS buffer; the last phase writes the
 // tile to dst.  The wrapped halo also supplies the x-wrapped lid densities of
 // the two top corners (cavity_opt1.py:206-213), so the result is bitwise that
 // of K single-step launches, while HBM sees one read and one write per
@@ -161,25 +161,40 @@ __device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *
     }
 }
 
-// One phase from LDS to LDS: halo-(H+1) values in, halo-H values out.
+// One phase in place in LDS: halo-(H+1) values in, halo-H values out, in the
+// same buffer.  Every thread first pulls its nodes (at most V = 2) and applies
+// the walls, which read only the buffer, into registers; after a barrier it
+// collides them and writes the smaller ring over the old one.  One buffer
+// instead of two halves the LDS of a block.
 template <typename T, int BC, int TR, int TC, int H, int BS>
-__device__ __forceinline__ void phase_lds(const T *in, T *out, int k0, int l0, int nx, int ny,
-                                          T omega, T u0) {
+__device__ __forceinline__ void phase_lds(T *buf, int k0, int l0, int nx, int ny, T omega, T u0) {
     constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
+    constexpr int V = (N + BS - 1) / BS;
+    static_assert(V <= 2, "an in-place LDS phase holds at most two nodes per thread");
+    T g[V][9];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
+        if (n >= N) continue;
         const int r = n / W, c = n - r * W;
         int k = k0 - H + r, l = l0 - H + c;
         k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
         l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
         const int m = (r + 1) * Wi + (c + 1);
-        T g[9];
-        pull_lds(in, Ni, Wi, m, g);
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
-        T rho, ux, uy;
-        collide_opt1(g, omega, rho, ux, uy);
+        pull_lds(buf, Ni, Wi, m, g[v]);
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+    }
+    __syncthreads();  // every read of the old ring is done
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
+        if (n >= N) continue;
+        T rho, ux, uy;
+        collide_opt1(g[v], omega, rho, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) buf[i * N + n] = g[v][i];
     }
 }
 
@@ -218,10 +233,8 @@ __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__re
                                                   T omega, T u0, T *__restrict__ rho_o,
                                                   T *__restrict__ ux_o, T *__restrict__ uy_o) {
     static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
-    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));  // halo K-1 (and K-3)
-    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;  // halo K-2
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));  // the phase-1 ring, the largest
     __shared__ T sa[9 * NA];
-    __shared__ T sb[9 * NB];
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // whole block: before any barrier
     const int tr = tile / ntc, tc = tile - tr * ntc;
@@ -229,19 +242,15 @@ __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__re
 
     phase_from_global<T, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0);
     __syncthreads();
-    if constexpr (K == 2) {
-        phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
-    } else if constexpr (K == 3) {
-        phase_lds<T, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+    if constexpr (K >= 4) {
+        phase_lds<T, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0);
         __syncthreads();
-        phase_to_global<T, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
-    } else {
-        phase_lds<T, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
-        __syncthreads();
-        phase_lds<T, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0);
-        __syncthreads();
-        phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
     }
+    if constexpr (K >= 3) {
+        phase_lds<T, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+    }
+    phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
 }
 
 // Streaming with cavity walls, no collide (stream_and_bounce_back).
@@ -510,12 +519,17 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
     X(9, 3, 8, 32, 512)       \
     X(10, 3, 12, 32, 512)     \
     X(11, 3, 8, 64, 512)      \
-    X(12, 4, 8, 32, 512)
+    X(12, 4, 8, 32, 512)      \
+    X(13, 3, 16, 32, 512)     \
+    X(14, 4, 12, 32, 512)     \
+    X(15, 4, 16, 32, 512)     \
+    X(16, 4, 10, 32, 512)     \
+    X(17, 3, 24, 32, 512)
 #define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
 constexpr int kNumMultiShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
-// Three steps on 12 x 32 tiles with 512 threads: the fastest FP64 shape measured
-// (83 GLUPS at 2048^2, 89 at 8192^2; 75.8 KB of LDS, so two blocks per CU).
-constexpr int kDefaultMultiShape = 10;
+// Four steps on 16 x 32 tiles with 512 threads: the fastest FP64 shape measured
+// (96 GLUPS at 2048^2, 100 at 8192^2; 60.2 KB of LDS, so two blocks per CU).
+constexpr int kDefaultMultiShape = 15;
 
 template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
