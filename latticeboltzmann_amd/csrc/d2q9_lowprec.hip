@@ -124,32 +124,49 @@ __device__ __forceinline__ void dev_phase_global(const typename Codec<FMT>::raw 
     }
 }
 
+// In place in the one LDS buffer: every thread pulls its nodes (at most two)
+// and applies the walls into registers, waits at a barrier, then collides,
+// rounds and writes the smaller ring over the old one.
 template <int FMT, typename CT, int BC, int TR, int TC, int H, int BS>
-__device__ __forceinline__ void dev_phase_lds(const typename Codec<FMT>::raw *in,
-                                              typename Codec<FMT>::raw *out, int k0, int l0,
-                                              int nx, int ny, CT omega, CT u0, CT scale,
-                                              CT inv_scale, int stochastic, uint32_t seed) {
+__device__ __forceinline__ void dev_phase_lds(typename Codec<FMT>::raw *buf, int k0, int l0, int nx,
+                                              int ny, CT omega, CT u0, CT scale, CT inv_scale,
+                                              int stochastic, uint32_t seed) {
     using C = Codec<FMT>;
     using R = typename C::raw;
     constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
+    constexpr int V = (N + BS - 1) / BS;
+    static_assert(V <= 2, "an in-place LDS phase holds at most two nodes per thread");
+    CT g[V][9];
+    int kv[V], lv[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
         const int r = n / W, c = n - r * W;
-        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+        kv[v] = wrap1(k0 - H + r, nx);
+        lv[v] = wrap1(l0 - H + c, ny);
+        if (n >= N) continue;
         const int m = (r + 1) * Wi + (c + 1);
         R raw[9];
-        pull_lds(in, Ni, Wi, m, raw);
-        CT g[9];
+        pull_lds(buf, Ni, Wi, m, raw);
 #pragma unroll
-        for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[i])) * inv_scale;
+        for (int i = 0; i < 9; ++i) g[v][i] = CT(C::dec(raw[i])) * inv_scale;
         if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls_dev<FMT, CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0, inv_scale);
-        CT ux, uy;
-        collide_dev<CT>(g, omega, ux, uy);
-        Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
-        store_values<FMT, CT>(g, scale, stochastic, dither, raw);
+            cavity_walls_dev<FMT, CT>(g[v], buf + m, int64_t(Ni), kv[v], lv[v], nx, ny, u0,
+                                      inv_scale);
+    }
+    __syncthreads();  // every read of the old ring is done
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = raw[i];
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
+        if (n >= N) continue;
+        CT ux, uy;
+        collide_dev<CT>(g[v], omega, ux, uy);
+        Dither dither(seed, uint64_t(kv[v]) * uint64_t(ny) + uint64_t(lv[v]));
+        R raw[9];
+        store_values<FMT, CT>(g[v], scale, stochastic, dither, raw);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) buf[i * N + n] = raw[i];
     }
 }
 
@@ -235,21 +252,33 @@ __device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::r
 }
 
 template <typename CT, int BC, int TR, int TC, int H, int BS>
-__device__ __forceinline__ void dev_phase_lds_ct(const CT *in, CT *out, int k0, int l0, int nx,
-                                                 int ny, CT omega, CT u0) {
+__device__ __forceinline__ void dev_phase_lds_ct(CT *buf, int k0, int l0, int nx, int ny, CT omega,
+                                                 CT u0) {
     constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
-    for (int n = threadIdx.x; n < N; n += BS) {
+    constexpr int V = (N + BS - 1) / BS;
+    static_assert(V <= 2, "an in-place LDS phase holds at most two nodes per thread");
+    CT g[V][9];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
+        if (n >= N) continue;
         const int r = n / W, c = n - r * W;
         const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
         const int m = (r + 1) * Wi + (c + 1);
-        CT g[9];
-        pull_lds(in, Ni, Wi, m, g);
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls_ct<CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
-        CT ux, uy;
-        collide_dev<CT>(g, omega, ux, uy);
+        pull_lds(buf, Ni, Wi, m, g[v]);
+        if constexpr (BC == LBM_BC_CAVITY)
+            cavity_walls_ct<CT>(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+    }
+    __syncthreads();  // every read of the old ring is done
 #pragma unroll
-        for (int i = 0; i < 9; ++i) out[i * N + n] = g[i];
+    for (int v = 0; v < V; ++v) {
+        const int n = threadIdx.x + v * BS;
+        if (n >= N) continue;
+        CT ux, uy;
+        collide_dev<CT>(g[v], omega, ux, uy);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) buf[i * N + n] = g[v][i];
     }
 }
 
@@ -297,10 +326,8 @@ __global__ __launch_bounds__(BS) void k_dev_stepk_once(
     int stochastic, uint32_t seed, uint32_t seed_stride, float *__restrict__ ux_o,
     float *__restrict__ uy_o) {
     static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
-    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
-    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));  // the phase-1 ring, the largest
     __shared__ CT sa[9 * NA];
-    __shared__ CT sb[9 * NB];
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // whole block: before any barrier
     const int tr = tile / ntc, tc = tile - tr * ntc;
@@ -310,22 +337,16 @@ __global__ __launch_bounds__(BS) void k_dev_stepk_once(
     dev_phase_global_ct<FMT, CT, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0,
                                                         inv_scale);
     __syncthreads();
-    if constexpr (K == 2) {
-        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
-                                                             scale, stochastic, last, ux_o, uy_o);
-    } else if constexpr (K == 3) {
-        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
+    if constexpr (K >= 4) {
+        dev_phase_lds_ct<CT, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0);
         __syncthreads();
-        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0,
-                                                             scale, stochastic, last, ux_o, uy_o);
-    } else {
-        dev_phase_lds_ct<CT, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0);
-        __syncthreads();
-        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0);
-        __syncthreads();
-        dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
-                                                             scale, stochastic, last, ux_o, uy_o);
     }
+    if constexpr (K >= 3) {
+        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0);
+        __syncthreads();
+    }
+    dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
+                                                         scale, stochastic, last, ux_o, uy_o);
 }
 
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
@@ -338,10 +359,8 @@ __global__ __launch_bounds__(BS) void k_dev_stepk(const typename Codec<FMT>::raw
                                                   float *__restrict__ uy_o) {
     using R = typename Codec<FMT>::raw;
     static_assert(K >= 2 && K <= 4, "2 to 4 steps per launch");
-    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
-    constexpr int NB = K >= 3 ? (TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
+    constexpr int NA = (TR + 2 * (K - 1)) * (TC + 2 * (K - 1));  // the phase-1 ring, the largest
     __shared__ R sa[9 * NA];
-    __shared__ R sb[9 * NB];
     const int tile = xcd_tile(ntiles);
     if (tile >= ntiles) return;  // whole block: before any barrier
     const int tr = tile / ntc, tc = tile - tr * ntc;
@@ -351,28 +370,19 @@ __global__ __launch_bounds__(BS) void k_dev_stepk(const typename Codec<FMT>::raw
     dev_phase_global<FMT, CT, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0, scale,
                                                      inv_scale, stochastic, seed);
     __syncthreads();
-    if constexpr (K == 2) {
-        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
-                                                          scale, inv_scale, stochastic, s1, ux_o,
-                                                          uy_o);
-    } else if constexpr (K == 3) {
-        dev_phase_lds<FMT, CT, BC, TR, TC, 1, BS>(sa, sb, k0, l0, nx, ny, omega, u0, scale,
-                                                  inv_scale, stochastic, s1);
+    if constexpr (K >= 4) {
+        dev_phase_lds<FMT, CT, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0, scale, inv_scale,
+                                                  stochastic, s1);
         __syncthreads();
-        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sb, dst, k0, l0, nx, ny, L, omega, u0,
-                                                          scale, inv_scale, stochastic, s2, ux_o,
-                                                          uy_o);
-    } else {
-        dev_phase_lds<FMT, CT, BC, TR, TC, 2, BS>(sa, sb, k0, l0, nx, ny, omega, u0, scale,
-                                                  inv_scale, stochastic, s1);
-        __syncthreads();
-        dev_phase_lds<FMT, CT, BC, TR, TC, 1, BS>(sb, sa, k0, l0, nx, ny, omega, u0, scale,
-                                                  inv_scale, stochastic, s2);
-        __syncthreads();
-        dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
-                                                          scale, inv_scale, stochastic, s3, ux_o,
-                                                          uy_o);
     }
+    if constexpr (K >= 3) {
+        dev_phase_lds<FMT, CT, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0, scale, inv_scale,
+                                                  stochastic, K == 4 ? s2 : s1);
+        __syncthreads();
+    }
+    dev_phase_to_global<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, scale,
+                                                      inv_scale, stochastic,
+                                                      K == 2 ? s1 : (K == 3 ? s2 : s3), ux_o, uy_o);
 }
 
 // FP64 populations (reference layout) -> stored deviations (any layout).
@@ -540,12 +550,18 @@ void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc
     X(0, 2, 8, 64, 256)         \
     X(1, 2, 16, 64, 512)        \
     X(2, 3, 12, 32, 512)        \
-    X(3, 3, 16, 64, 512)        \
+    X(3, 3, 16, 32, 512)        \
     X(4, 4, 8, 64, 512)         \
-    X(5, 4, 16, 64, 512)
+    X(5, 4, 12, 32, 512)        \
+    X(6, 4, 16, 32, 512)        \
+    X(7, 4, 24, 32, 512)        \
+    X(8, 4, 32, 32, 1024)
 #define LBM_DEV_COUNT_MULTI(id, k, tr, tc, bs) +1
 constexpr int kNumDevMultiShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
-constexpr int kDefaultDevMultiShape = 2;
+// Four steps on 32 x 32 tiles with 1024 threads: the fastest measured for FP32
+// and FP16 storage (FP16 stored every 4 steps 220 GLUPS, FP32 storage 207 GLUPS
+// at 8192^2).
+constexpr int kDefaultDevMultiShape = 8;
 
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t seed_stride,
@@ -566,8 +582,7 @@ int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t se
         // unrounded values in the compute type: an FP64 tile can outgrow the
         // 160 KB of LDS of a gfx950 CU, and such shapes are refused
         constexpr int64_t NA = int64_t(TR + 2 * (K - 1)) * (TC + 2 * (K - 1));
-        constexpr int64_t NB = K >= 3 ? int64_t(TR + 2 * (K - 2)) * (TC + 2 * (K - 2)) : 1;
-        constexpr int64_t lds = 9 * (NA + NB) * int64_t(sizeof(CT));
+        constexpr int64_t lds = 9 * NA * int64_t(sizeof(CT));
         if constexpr (lds <= 160 * 1024) {
             hipLaunchKernelGGL((k_dev_stepk_once<FMT, CT, BC, MOM, K, TR, TC, BS>),
                                dim3(unsigned(nblocks)), dim3(BS), 0, s, static_cast<const R *>(src),

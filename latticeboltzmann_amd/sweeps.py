@@ -10,7 +10,7 @@
   the FP64 collide here: an FP32 collide stores unrounded FP32 results and
   shares the bias of the 16-bit FP32 collide).
 * The FP16 cavity is also run with the FP64 collide, and FP16/BF16 storing
-  every 3 steps (intermediate steps unrounded on chip).
+  every 3 and every 4 steps (intermediate steps unrounded on chip).
 
 Usage::
 
@@ -43,7 +43,7 @@ def _timed(fn):
 
 
 def cavity_precision(n=8192, re=1000.0, u=0.1, steps=50000, formats=FORMATS, log=None,
-                     store_every=(3,)):
+                     store_every=(3, 4)):
     om = omega_for_cavity(n, re, u)
     ref = Lattice(n, n, om, "cavity", u, torch.float64, variant="auto")
     m0 = ref.mass()
