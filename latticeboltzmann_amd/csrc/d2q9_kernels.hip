@@ -524,12 +524,16 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
     X(14, 4, 12, 32, 512)     \
     X(15, 4, 16, 32, 512)     \
     X(16, 4, 10, 32, 512)     \
-    X(17, 3, 24, 32, 512)
+    X(17, 3, 24, 32, 512)     \
+    X(18, 4, 20, 32, 512)     \
+    X(19, 4, 24, 28, 512)
 #define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
 constexpr int kNumMultiShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
-// Four steps on 16 x 32 tiles with 512 threads: the fastest FP64 shape measured
-// (96 GLUPS at 2048^2, 100 at 8192^2; 60.2 KB of LDS, so two blocks per CU).
-constexpr int kDefaultMultiShape = 15;
+// Four steps on 20 x 32 tiles with 512 threads: the fastest FP64 shape measured
+// (100-102 GLUPS at 2048^2, 105-106 at 8192^2; 71.1 KB of LDS, so two blocks
+// per CU).  Its four rings (26x38, 24x36, 22x34, 20x32 nodes) take two passes
+// of the block each, with few idle lanes in the second.
+constexpr int kDefaultMultiShape = 18;
 
 template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,

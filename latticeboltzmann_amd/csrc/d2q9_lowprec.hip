@@ -555,13 +555,16 @@ void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc
     X(5, 4, 12, 32, 512)        \
     X(6, 4, 16, 32, 512)        \
     X(7, 4, 24, 32, 512)        \
-    X(8, 4, 32, 32, 1024)
+    X(8, 4, 32, 32, 1024)       \
+    X(9, 4, 40, 32, 1024)       \
+    X(10, 4, 46, 32, 1024)
 #define LBM_DEV_COUNT_MULTI(id, k, tr, tc, bs) +1
 constexpr int kNumDevMultiShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
-// Four steps on 32 x 32 tiles with 1024 threads: the fastest measured for FP32
-// and FP16 storage (FP16 stored every 4 steps 220 GLUPS, FP32 storage 207 GLUPS
-// at 8192^2).
-constexpr int kDefaultDevMultiShape = 8;
+// Four steps on 46 x 32 tiles with 1024 threads: the fastest measured for FP32
+// and FP16 storage (at 8192^2, FP16 stored every 4 steps 230 GLUPS and FP32
+// storage 224 in a same-run comparison).  Its four rings (52x38 .. 46x32 nodes)
+// take two passes of the block each, with few idle lanes in the second.
+constexpr int kDefaultDevMultiShape = 10;
 
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t seed_stride,

@@ -102,7 +102,7 @@ class DeviationLattice:
         self.init_equilibrium()
 
     # shapes tried first for each K when storing every K steps
-    _STORE_EVERY_SHAPES = {2: 1, 3: 3, 4: 8}
+    _STORE_EVERY_SHAPES = {2: 1, 3: 3, 4: 10}
 
     def _set_store_every(self, K, tune):
         """Store every K steps: runs of K steps through one multi-step launch
