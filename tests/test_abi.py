@@ -108,3 +108,24 @@ def test_dropin_imports_as_top_level_pylb():
                          capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     assert out.stdout.strip() == "ok"
+
+
+def test_multi_step_shape_tables_without_gpu():
+    # every multi-step shape reports 2..4 steps and a smallest grid of at
+    # least its tile plus the halo; the defaults are shapes of the tables, and
+    # each store-every-K shape of DeviationLattice steps K times per launch
+    from latticeboltzmann_amd import kernels
+    from latticeboltzmann_amd.precision import DeviationLattice, multi_info
+    from latticeboltzmann_amd import _native
+    lib = _native.load()
+    for count, info, default in (
+            (lib.lbm_step_multi_variant_count(), kernels.step_multi_info,
+             lib.lbm_step_multi_default_variant()),
+            (lib.lbm_dev_multi_variant_count(), multi_info, lib.lbm_dev_multi_default_variant())):
+        assert 0 <= default < count
+        assert info(-1) == info(default)
+        for v in range(count):
+            K, mnx, mny = info(v)
+            assert 2 <= K <= 4 and mnx >= 2 * K + 1 and mny >= 2 * K + 1, (v, K, mnx, mny)
+    for K, shape in DeviationLattice._STORE_EVERY_SHAPES.items():
+        assert multi_info(shape)[0] == K, (K, shape)
