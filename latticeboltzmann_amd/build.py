@@ -50,21 +50,36 @@ def hipcc():
     return exe
 
 
-def build(verbose=False):
-    """Compile every HIP unit for gfx950 and link ``liblbm_d2q9.so``."""
+def build(verbose=False, jobs=None):
+    """Compile every HIP unit for gfx950 and link ``liblbm_d2q9.so``.
+
+    The units compile concurrently (at most ``jobs`` at a time, default one
+    per unit up to the CPU count); a failing unit raises after the others
+    have finished."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     cc = hipcc()
-    objs = []
+    cmds, objs = [], []
     for unit, flags in UNITS.items():
         src = os.path.join(CSRC, unit)
         if not os.path.exists(src):
             continue
         obj = os.path.join(BUILD_DIR, unit.replace(".hip", ".o"))
-        cmd = [cc, *COMMON, *flags, "-c", src, "-o", obj]
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append([cc, *COMMON, *flags, "-c", src, "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    failed = None
+    running = []
+    for cmd in cmds + [None] * jobs:
+        while running and (cmd is None or len(running) >= jobs):
+            c, p = running.pop(0)
+            if p.wait() != 0 and failed is None:
+                failed = subprocess.CalledProcessError(p.returncode, c)
+        if cmd is not None:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            running.append((cmd, subprocess.Popen(cmd)))
+    if failed is not None:
+        raise failed
     tmp = LIB_PATH + ".tmp"
     cmd = [cc, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp]
     if verbose:
