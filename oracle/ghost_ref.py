@@ -34,8 +34,10 @@ lid term of a state at rest (wall density 1); and by the analytic Couette and
 Poiseuille profiles (``PoiseuilleFlow.py:118-119, 152-154``) and the reversal
 of the Poiseuille flow when inlet and outlet densities swap.  The obstacle's
 right face writes NW from SE (``obstacle_canal.py:437-438``) where a
-bounce-back would write SE from NW; that rule is restated as written and no
-pulse test covers it.
+bounce-back would write SE from NW; that rule is restated as written, no pulse
+test covers it, and an index-encoded known-answer table of the script's twelve
+assignments (``obstacle_canal.py:421-458``) checks every cell the obstacle
+update writes.
 """
 
 import numpy as np

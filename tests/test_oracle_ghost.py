@@ -102,3 +102,40 @@ def test_poiseuille_profile():
     G, (rho, ux2, uy2) = g.poiseuille(42, 22, 0.999, 1.001, 0.5, 200)
     G, (rho, ux1, uy1) = g.poiseuille(42, 22, 1.001, 0.999, 0.5, 200)
     assert np.all(ux1[21, 1:-1] > 0) and np.allclose(ux2[21, 1:-1], -ux1[21, 1:-1], rtol=1e-2)
+
+
+def test_obstacle_faces_index_encoded_kat():
+    # Every population holds a code of its own (channel, x, y); after the
+    # obstacle update each written cell must hold the code of the source cell
+    # that the script's assignment names, and nothing else may change.  The
+    # table is the script's twelve assignments as data (destination channel,
+    # column, rows <- source channel, column, rows), read off
+    # obstacle_canal.py:421-458, including the right and top faces writing NW
+    # (6) from SE (8) where a bounce-back pair would take SW or NE.
+    nx, ny, sx, sy, ex, ey = 12, 10, 4, 3, 7, 6
+    c, x, y = np.meshgrid(np.arange(9), np.arange(nx), np.arange(ny), indexing="ij")
+    g0 = (10000 * c + 100 * x + y).astype(np.float64)
+    rows, cols = slice(sy, ey), slice(sx, ex)
+    table = [
+        ((3, sx - 1, rows), (1, sx, rows)),     # :421-422
+        ((7, sx - 1, rows), (5, sx, rows)),     # :424-425
+        ((6, sx - 1, rows), (8, sx, rows)),     # :427-428
+        ((1, ex + 1, rows), (3, ex, rows)),     # :431-432
+        ((5, ex + 1, rows), (7, ex, rows)),     # :434-435
+        ((6, ex + 1, rows), (8, ex, rows)),     # :437-438
+        ((2, cols, ey + 1), (4, cols, ey)),     # :441-442
+        ((5, cols, ey + 1), (7, cols, ey)),     # :444-445
+        ((6, cols, ey + 1), (8, cols, ey)),     # :447-448
+        ((4, cols, sy - 1), (2, cols, sy)),     # :451-452
+        ((7, cols, sy - 1), (5, cols, sy)),     # :454-455
+        ((8, cols, sy - 1), (6, cols, sy)),     # :457-458
+    ]
+    expected = g0.copy()
+    for dst, src in table:
+        expected[dst] = g0[src]
+    got = g0.copy()
+    g.apply_obstacle(got, sx, sy, ex, ey)
+    assert np.array_equal(got, expected)
+    # the written cells are exactly the table's destinations
+    changed = np.argwhere(got != g0)
+    assert len(changed) == 3 * 2 * (ey - sy) + 3 * 2 * (ex - sx)
