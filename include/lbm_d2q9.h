@@ -167,6 +167,19 @@ int lbm_sum_f32(const float *x, int64_t n, double *work, double *out, void *stre
 int lbm_velocity_residual_f64(const double *f, int64_t nx, int64_t ny, int layout,
                               double *ux_prev, double *uy_prev, double *work,
                               double *out, void *stream);
+/* The same residual without a workspace argument and without allocating:
+ * its partial maxima go to a library-owned device array of
+ * lbm_noalloc_scratch_bytes() bytes (8 KiB; one per GPU, held until the
+ * process ends; every call writes all of it before reading it, so nothing
+ * needs clearing).  Accepts every grid lbm_velocity_residual_f64 accepts
+ * (LBM_E_SIZE beyond them) and gives bit-identical results.  Calls that
+ * overlap on one GPU (other streams or threads) share that array and race.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight convergence loops).           */
+int64_t lbm_noalloc_scratch_bytes(void);
+int lbm_velocity_residual_f64_noalloc(const double *f, int64_t nx, int64_t ny, int layout,
+                                      double *ux_prev, double *uy_prev, double *out,
+                                      void *stream);
 
 /* ---- reduced-precision storage ---------------------------------------------------
  * Populations stored as scaled deviations d_i = scale * (f_i - w_i) in FP32,

@@ -88,6 +88,8 @@ SIGNATURES = {
     "lbm_sum_f64": (_i32, [_vp, _i64, _vp, _vp, _vp]),
     "lbm_sum_f32": (_i32, [_vp, _i64, _vp, _vp, _vp]),
     "lbm_velocity_residual_f64": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp]),
+    "lbm_noalloc_scratch_bytes": (_i64, []),
+    "lbm_velocity_residual_f64_noalloc": (_i32, [_vp, _i64, _i64, _i32, _vp, _vp, _vp, _vp]),
     # reduced-precision storage
     "lbm_dev_variant_count": (_i32, []),
     "lbm_dev_encode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),

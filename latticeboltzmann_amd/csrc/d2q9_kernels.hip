@@ -13,6 +13,8 @@
 // rows k-1, k, k+1 shifted by at most one element, so the reads stay
 // coalesced and every population is read exactly once per step.
 
+#include <mutex>
+
 #include "d2q9_grid.hpp"
 #include "d2q9_node.hpp"
 #include "d2q9_reduce.hpp"
@@ -673,6 +675,40 @@ int equilibrium_impl(const T *rho, const T *ux, const T *uy, T *f, int64_t n, vo
     return launch_status();
 }
 
+// Scratch of the allocation-free residual (lbm_velocity_residual_f64_noalloc):
+// the kReduceBlocks per-block partial maxima, one copy per GPU for the life
+// of the process.  Host code never passes `scratch` itself to a kernel (on
+// the host it is only a stand-in for the symbol); it passes the device
+// address that hipGetSymbolAddress returns for the current device.
+constexpr int kScratchDoubles = kReduceBlocks;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex scratch_mutex;
+static double *scratch_ptr[kMaxScratchDevices];  // per device, set once
+
+// Device address of `scratch` on the current device, looked up once per
+// device (with that device current) and kept; nullptr on a HIP error.
+double *scratch_for_current_device(int *err) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxScratchDevices) {
+        *err = e != hipSuccess ? int(e) : LBM_E_ARG;
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *err = e != hipSuccess ? int(e) : LBM_E_NULL;
+            return nullptr;
+        }
+        scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    return scratch_ptr[dev];
+}
+
 void launch_sum_pass2(const double *partial, int np, double bias, double *out, hipStream_t s) {
     hipLaunchKernelGGL(k_sum_pass2, dim3(1), dim3(kBlock), 0, s, partial, np, bias, out);
 }
@@ -849,6 +885,27 @@ int lbm_velocity_residual_f64(const double *f, int64_t nx, int64_t ny, int layou
     if (!f || !ux_prev || !uy_prev || !work || !out) return LBM_E_NULL;
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
     if (!valid_layout(layout)) return LBM_E_ARG;
+    hipStream_t s = as_stream(stream);
+    hipLaunchKernelGGL(k_residual_pass1, dim3(kReduceBlocks), dim3(kBlock), 0, s, f, int(nx),
+                       int(ny), make_layout(layout, nx, ny), ux_prev, uy_prev, work);
+    hipLaunchKernelGGL(k_max_pass2, dim3(1), dim3(kBlock), 0, s, work, kReduceBlocks, out);
+    return launch_status();
+}
+
+int64_t lbm_noalloc_scratch_bytes(void) { return int64_t(kScratchDoubles) * int64_t(sizeof(double)); }
+
+int lbm_velocity_residual_f64_noalloc(const double *f, int64_t nx, int64_t ny, int layout,
+                                      double *ux_prev, double *uy_prev, double *out,
+                                      void *stream) {
+    if (!f || !ux_prev || !uy_prev || !out) return LBM_E_NULL;
+    if (!valid_grid(nx, ny) || nx > INT32_MAX || ny > INT32_MAX) return LBM_E_SIZE;
+    if (!valid_layout(layout)) return LBM_E_ARG;
+    // pass 1 writes every one of its kReduceBlocks partials before pass 2
+    // reads them, so the scratch needs no clearing between calls
+    static_assert(kReduceBlocks <= kScratchDoubles, "partials must fit the scratch");
+    int err = LBM_OK;
+    double *work = scratch_for_current_device(&err);
+    if (!work) return err;
     hipStream_t s = as_stream(stream);
     hipLaunchKernelGGL(k_residual_pass1, dim3(kReduceBlocks), dim3(kBlock), 0, s, f, int(nx),
                        int(ny), make_layout(layout, nx, ny), ux_prev, uy_prev, work);
