@@ -4,8 +4,11 @@ Every launch shape (``variant``) of a step kernel computes the same bits; they
 differ only in threads per block, nodes per thread and cache hints, and the
 fastest one depends on the grid and the storage type.  ``best_variant`` times
 each shape on a scratch grid of the requested size class and caches the winner
-per (device, kernel, dtype, layout, boundary kind, size class) in a JSON file
-(``$LBM_TUNE_CACHE`` or ``~/.cache/latticeboltzmann_amd/tune.json``).
+per (library build, device, kernel, dtype, layout, boundary kind, size class)
+in a JSON file (``$LBM_TUNE_CACHE`` or
+``~/.cache/latticeboltzmann_amd/tune.json``).  The build is the content hash
+of ``liblbm_d2q9.so``, so a rebuilt library (other kernels, another shape
+table) is tuned afresh instead of reusing choices measured on another build.
 
 Size classes round each side up to a power of two, so a 1000 x 1000 grid and a
 1024 x 1024 grid share one entry.
@@ -80,7 +83,8 @@ def best_variant(kernel, nx, ny, dtype, layout, bc, make_launch, variants, reps=
     the launch times to the times compared (e.g. per step).
     """
     dev = torch.cuda.get_device_name(torch.cuda.current_device())
-    key = "|".join([dev, kernel, str(dtype), str(layout), str(bc), size_class(nx, ny)])
+    key = "|".join([nat.build_id(), dev, kernel, str(dtype), str(layout), str(bc),
+                    size_class(nx, ny)])
     with _LOCK:
         if key in _MEMO:
             return _MEMO[key]
@@ -123,7 +127,7 @@ def step_variant(nx, ny, dtype, layout, bc, reps=10, use_cache=True):
         return launch
 
     n_var = nat.load().lbm_step_variant_count()
-    return best_variant("step%d" % n_var, nx, ny, dtype, layout, bc, make_launch, range(n_var),
+    return best_variant("step", nx, ny, dtype, layout, bc, make_launch, range(n_var),
                         reps=reps, use_cache=use_cache)
 
 
@@ -157,7 +161,5 @@ def multi_variant(nx, ny, dtype, layout, bc, reps=10, use_cache=True):
     def per_step(times):
         return {v: t / info[v][0] for v, t in times.items()}
 
-    # the table size in the key keeps a cache written for another shape table
-    # from naming the wrong shape
-    return best_variant("step_multi%d" % n_var, nx, ny, dtype, layout, bc, make_launch, fits,
+    return best_variant("step_multi", nx, ny, dtype, layout, bc, make_launch, fits,
                         reps=reps, use_cache=use_cache, normalize=per_step)

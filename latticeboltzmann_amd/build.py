@@ -13,7 +13,8 @@ flags and the objects are linked into one shared library next to this file.
   kernels ("Cannot select: v5i32 = bitcast", from packing the nine 16-bit
   outputs of a node); the kernels are bandwidth-bound, so nothing is lost.
 
-Usage: ``python -m latticeboltzmann_amd.build`` (or ``build()`` from Python).
+Usage: ``python -m latticeboltzmann_amd.build [--changed] [-v]`` (or ``build()``
+from Python).
 """
 
 import os
@@ -50,12 +51,24 @@ def hipcc():
     return exe
 
 
-def build(verbose=False, jobs=None):
+def _stale(obj, src):
+    """True when ``obj`` is older than its unit, any shared header or the C ABI header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src, os.path.join(HERE, "..", "include", "lbm_d2q9.h")]
+    deps += [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith((".hpp", ".h"))]
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) > t for d in deps if os.path.exists(d))
+
+
+def build(verbose=False, jobs=None, changed_only=False):
     """Compile every HIP unit for gfx950 and link ``liblbm_d2q9.so``.
 
     The units compile concurrently (at most ``jobs`` at a time, default one
     per unit up to the CPU count); a failing unit raises after the others
-    have finished."""
+    have finished.  ``changed_only`` recompiles only units whose object is
+    older than the unit or a header (a development shortcut; the default
+    compiles everything)."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     cc = hipcc()
     cmds, objs = [], []
@@ -64,9 +77,10 @@ def build(verbose=False, jobs=None):
         if not os.path.exists(src):
             continue
         obj = os.path.join(BUILD_DIR, unit.replace(".hip", ".o"))
-        cmds.append([cc, *COMMON, *flags, "-c", src, "-o", obj])
+        if not changed_only or _stale(obj, src):
+            cmds.append([cc, *COMMON, *flags, "-c", src, "-o", obj])
         objs.append(obj)
-    jobs = jobs or min(len(cmds), os.cpu_count() or 1)
+    jobs = jobs or max(1, min(len(cmds), os.cpu_count() or 1))
     failed = None
     running = []
     for cmd in cmds + [None] * jobs:
@@ -90,5 +104,5 @@ def build(verbose=False, jobs=None):
 
 
 if __name__ == "__main__":
-    build(verbose="-v" in sys.argv)
+    build(verbose="-v" in sys.argv, changed_only="--changed" in sys.argv)
     print(LIB_PATH)

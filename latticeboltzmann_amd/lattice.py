@@ -227,6 +227,27 @@ class StepGraph:
         swaps = lat.launches(self.n) if hasattr(lat, "launches") else self.n
         self.enabled = bool(enabled) and swaps % 2 == 0
         self.graph = None
+        self._state_ptr = None
+
+    def _state(self):
+        """(current buffer, other buffer) attribute names of the lattice."""
+        return ("d", "_e") if hasattr(self.lat, "_e") else ("f", "_g")
+
+    def _realign(self):
+        """The graph reads and writes the buffers it was captured with.  An odd
+        number of eager launches since the capture (``lat.step`` with a count
+        that is not a multiple of the replayed one) leaves the state in the
+        other buffer: copy it back and swap the lattice's references, so the
+        replay steps the current state."""
+        cur, other = self._state()
+        a, b = getattr(self.lat, cur), getattr(self.lat, other)
+        if a.data_ptr() == self._state_ptr:
+            return
+        if b.data_ptr() != self._state_ptr:
+            raise RuntimeError("StepGraph: the lattice's buffers changed since the capture")
+        b.copy_(a)
+        setattr(self.lat, cur, b)
+        setattr(self.lat, other, a)
 
     def __enter__(self):
         if not self.enabled:
@@ -236,6 +257,7 @@ class StepGraph:
         self.side = torch.cuda.Stream(device=self.lat.device)
         self.side.wait_stream(torch.cuda.current_stream(self.lat.device))
         g = ctypes.c_void_p()
+        self._state_ptr = getattr(self.lat, self._state()[0]).data_ptr()
         with torch.cuda.stream(self.side):
             nat.call("lbm_graph_begin", nat.stream_ptr())
             try:
@@ -248,6 +270,7 @@ class StepGraph:
         self.graph = g
 
         def replay():
+            self._realign()
             nat.call("lbm_graph_launch", self.graph, nat.stream_ptr())
             self.lat.time += self.n
         return replay

@@ -102,6 +102,8 @@ def test_dev_multi_step_equals_single_steps(gpu, fmt, compute, rounding):
     for bc in ("cavity", "periodic"):
         for v in range(nvar):
             K, mnx, mny = multi_info(v)
+            if compute == "f64" and K > 4:
+                continue  # the library refuses FP64-collide shapes beyond four steps
             nx, ny = mnx + 5, mny + 37
             print("dev multi", fmt, compute, rounding, bc, v, K, nx, ny, flush=True)
             rho = 1 + 0.01 * RNG.standard_normal((nx, ny))
@@ -148,3 +150,24 @@ def test_multi_step_large_ragged_grid(gpu, bc):
         assert torch.equal(a, b), v
         del a, b
         torch.cuda.empty_cache()
+
+
+def test_step_graph_replay_after_odd_eager_steps(gpu):
+    # A captured graph reads the buffer it was captured with; eager steps in
+    # between that leave the state in the other buffer must not make the next
+    # replay step a stale state.
+    from latticeboltzmann_amd.lattice import Lattice, StepGraph
+    omega = 1.529988
+    lat = Lattice(64, 64, omega, "cavity", 0.1, torch.float64)
+    ref = Lattice(64, 64, omega, "cavity", 0.1, torch.float64, multi_step=False)
+    assert lat.launches(8) % 2 == 0
+    with StepGraph(lat, 8) as advance:
+        assert lat.launches(8) % 2 == 0
+        advance()
+        lat.step(1)  # one single-step launch: the state moves to the other buffer
+        advance()
+        lat.step(3)
+        advance()
+    ref.step(8 + 1 + 8 + 3 + 8)
+    assert lat.time == ref.time == 28
+    assert np.array_equal(lat.populations(), ref.populations())
