@@ -1,8 +1,11 @@
-"""Host-side logic that needs no GPU: sweep draws, tile tables, table layouts."""
+"""Host-side logic that needs no GPU: sweep draws, tile tables, table layouts,
+checkpoint files."""
 
 import ctypes
 
 import numpy as np
+import pytest
+import torch
 
 
 def test_sweep_draws_are_reproducible():
@@ -78,3 +81,68 @@ def test_deviation_lattice_launch_count():
     lat = _bare(DeviationLattice, multi_variant=2, multi_k=3)
     assert [lat.launches(n) for n in (1, 3, 7, 9)] == [1, 1, 3, 3]
     assert _bare(DeviationLattice, multi_variant=None, multi_k=1).launches(7) == 7
+
+
+class _FakeLattice:
+    """Host stand-in with a Lattice's checkpointed attributes (CPU tensors)."""
+
+    def __init__(self, nx=4, ny=6, omega=1.2, dtype=torch.float64):
+        self.nx, self.ny, self.omega, self.bc, self.u0, self.layout = nx, ny, omega, "cavity", 0.1, 1
+        self.f = torch.zeros((nx, 9, ny), dtype=dtype)
+        self.time = 0
+
+
+class _FakeDeviationLattice(_FakeLattice):
+    def __init__(self, fmt=2, **kw):
+        super().__init__(**kw)
+        del self.f
+        self.fmt, self.scale, self.rounding, self.seed, self.compute, self.store_every = \
+            fmt, 1024.0, 1, 7, 0, 4
+        raw = torch.float32 if fmt == 0 else torch.int16
+        self.d = torch.zeros((self.nx, 9, self.ny), dtype=raw)
+
+
+def test_checkpoint_round_trip_and_parameter_checks(tmp_path):
+    from latticeboltzmann_amd import checkpoint
+    rng = np.random.default_rng(5)
+    a = _FakeLattice()
+    a.f.copy_(torch.from_numpy(rng.standard_normal(a.f.shape)))
+    a.f[0, 0, 0] = float("nan")  # every bit, NaN payloads included
+    a.time = 1234
+    p = str(tmp_path / "lat.npz")
+    checkpoint.save(a, p)
+    b = _FakeLattice()
+    checkpoint.restore(b, p)
+    assert b.time == 1234
+    assert np.array_equal(b.f.numpy().view(np.uint64), a.f.numpy().view(np.uint64))
+    for attr, val in (("omega", 1.3), ("bc", "periodic"), ("layout", 0)):
+        c = _FakeLattice()
+        setattr(c, attr, val)
+        with pytest.raises(ValueError):
+            checkpoint.restore(c, p)
+    with pytest.raises(ValueError):  # grid shape
+        checkpoint.restore(_FakeLattice(nx=5), p)
+    with pytest.raises(ValueError):  # storage type
+        checkpoint.restore(_FakeLattice(dtype=torch.float32), p)
+    with pytest.raises(ValueError):  # kind
+        checkpoint.restore(_FakeDeviationLattice(), p)
+
+
+def test_checkpoint_deviation_words(tmp_path):
+    from latticeboltzmann_amd import checkpoint
+    a = _FakeDeviationLattice()
+    a.d.copy_(torch.arange(a.d.numel(), dtype=torch.int16).view(a.d.shape))
+    a.time = 48
+    p = str(tmp_path / "dev.npz")
+    checkpoint.save(a, p)
+    b = _FakeDeviationLattice()
+    checkpoint.restore(b, p)
+    assert b.time == 48 and torch.equal(a.d, b.d)
+    c = _FakeDeviationLattice()
+    c.seed = 8  # another dither stream: not the same run
+    with pytest.raises(ValueError):
+        checkpoint.restore(c, p)
+    d = _FakeDeviationLattice()
+    d.store_every = 1
+    with pytest.raises(ValueError):
+        checkpoint.restore(d, p)
