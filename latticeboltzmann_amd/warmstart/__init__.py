@@ -12,7 +12,8 @@ converged field (``evaluate``).  The reference repository has no learned
 counterpart.
 """
 
-from .controller import Controller
+from .controller import Controller, WarmStartUnavailable
 from .features import GRID, RE_RANGE, U_RANGE, features, pooled_target
 
-__all__ = ["Controller", "GRID", "RE_RANGE", "U_RANGE", "features", "pooled_target"]
+__all__ = ["Controller", "WarmStartUnavailable", "GRID", "RE_RANGE", "U_RANGE", "features",
+           "pooled_target"]

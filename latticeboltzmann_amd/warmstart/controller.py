@@ -16,10 +16,20 @@ threshold.  A warm start is abandoned for a cold start when
   value for ``patience`` consecutive checks;
 * it runs over budget: more than ``budget`` times ``cold_estimate`` steps
   (when an estimate of the cold-start cost is given).
+
+A stepper that cannot offer a warm start at all raises
+``WarmStartUnavailable`` from ``init_warm``; the run is then a cold start
+reported as a fallback.
 """
 
 import math
 import time
+
+
+class WarmStartUnavailable(Exception):
+    """Raised by a stepper's ``init_warm`` when it has no prediction to offer
+    (for instance, Re or U outside the range the surrogate was trained on);
+    the controller then runs the cold start."""
 
 
 class Controller:
@@ -55,7 +65,12 @@ class Controller:
         out = {"warm": bool(warm), "fell_back": False, "reason": None, "warm_steps": 0,
                "cold_steps": 0}
         if warm:
-            stepper.init_warm()
+            try:
+                stepper.init_warm()
+            except WarmStartUnavailable:
+                out["fell_back"], out["reason"] = True, "no warm start"
+                warm = False
+        if warm:
             limit = self.max_steps
             if cold_estimate:
                 limit = min(limit, max(self.every, int(self.budget * cold_estimate)))

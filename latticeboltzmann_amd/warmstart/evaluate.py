@@ -10,10 +10,17 @@ difference max and RMS |u_warm - u_cold| / U.  With ``--pooled`` a third run
 starts from the cold result itself averaged onto the surrogate's grid: the
 best any predictor on that grid could do.
 
+With several sizes (``--n 256,512,1024``) each cavity's N is drawn with its
+(Re, U) from the seed, uniformly over the sizes.  Cavities whose cold start
+does not converge (the BGK lattice diverges or stalls above the threshold)
+have no yardstick: they are reported and left out of the summary.
+
 Usage::
 
     python -m latticeboltzmann_amd.warmstart.evaluate --weights w.npz --count 40 --seed 1000 \
         --n 256 --out eval.json
+    python -m latticeboltzmann_amd.warmstart.evaluate --weights default --n 256,512,1024 \
+        --re-range 100 5000 --count 60 --seed 3000
 """
 
 import argparse
@@ -25,9 +32,9 @@ import numpy as np
 import torch
 
 from .controller import Controller
-from .data import draw_cavities
+from .data import draw_cavities, draw_config_cavities
 from .features import pooled_target
-from .solve import DEFAULT_WEIGHTS, CavityStepper, Predictor
+from .solve import DEFAULT_WEIGHTS, CavityStepper, Predictor, load_default
 
 
 class _Fixed:
@@ -51,6 +58,8 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic"):
         cold = ctl.run(st, warm=False)
         fc = tuple(t.clone() for t in st.fields())
     rec["cold"] = {k: cold[k] for k in ("steps", "seconds", "reason")}
+    if cold["reason"] != "converged":
+        return rec  # no converged yardstick to judge a warm start against
     legs = [("warm", pred)]
     if pooled:
         m = pred.grid if pred else 128
@@ -68,16 +77,29 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic"):
 
 
 def summarize(records, legs=("warm",)):
-    out = {"cavities": len(records)}
+    out = {"cavities": len(records),
+           "cold_not_converged": [{"n": r["n"], "re": r["re"], "u": r["u"],
+                                   "reason": r["cold"]["reason"], "steps": r["cold"]["steps"]}
+                                  for r in records if r["cold"]["reason"] != "converged"]}
     for leg in legs:
         rs = [r[leg] for r in records if leg in r]
         if not rs:
             continue
+        out.setdefault("by_n", {})
+        for n in sorted({r["n"] for r in records if leg in r}):
+            sub = [r[leg] for r in records if leg in r and r["n"] == n]
+            out["by_n"]["%s_%d" % (leg, n)] = {
+                "cavities": len(sub),
+                "median_time_speedup": float(np.median([r["time_speedup"] for r in sub])),
+                "median_step_speedup": float(np.median([r["step_speedup"] for r in sub])),
+                "diff_max_worst": float(max(r["diff_max"] for r in sub)),
+                "fallbacks": int(sum(r["fell_back"] for r in sub))}
         out[leg] = {
             "median_step_speedup": float(np.median([r["step_speedup"] for r in rs])),
             "median_time_speedup": float(np.median([r["time_speedup"] for r in rs])),
             "total_time_speedup": float(sum(r["cold"]["seconds"] for r in records if leg in r)
                                         / sum(r["seconds"] for r in rs)),
+            "p10_time_speedup": float(np.percentile([r["time_speedup"] for r in rs], 10)),
             "min_time_speedup": float(min(r["time_speedup"] for r in rs)),
             "fallbacks": int(sum(r["fell_back"] for r in rs)),
             "diff_max_median": float(np.median([r["diff_max"] for r in rs])),
@@ -89,10 +111,13 @@ def summarize(records, legs=("warm",)):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS,
+                    help="an .npz of one network, or 'default' for the shipped set")
     ap.add_argument("--count", type=int, default=40)
     ap.add_argument("--seed", type=int, default=1000)
-    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--n", default="256", help="lattice size, or a comma list to draw from")
+    ap.add_argument("--re-range", type=float, nargs=2, default=None,
+                    help="Re range of the draw (default: the predictor's)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-neq", action="store_true")
     ap.add_argument("--pooled", action="store_true")
@@ -100,13 +125,20 @@ def main(argv=None):
     ap.add_argument("--time-limit", type=float, default=None)
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
-    pred = Predictor.load(a.weights)
+    pred = load_default() if a.weights == "default" else Predictor.load(a.weights)
     ctl = Controller(tol=a.tol)
+    sizes = [int(x) for x in str(a.n).split(",")]
+    re_range = tuple(a.re_range) if a.re_range else pred.re_range
+    if len(sizes) == 1:
+        cases = [(sizes[0], re, u) for re, u in
+                 draw_cavities(a.count, a.seed, re_range, pred.u_range)]
+    else:
+        cases = draw_config_cavities(a.count, a.seed, sizes, re_range, pred.u_range)
     records, t0 = [], time.time()
-    for i, (re, u) in enumerate(draw_cavities(a.count, a.seed, pred.re_range, pred.u_range)):
+    for n, re, u in cases:
         if a.time_limit and time.time() - t0 > a.time_limit:
             break
-        rec = evaluate_one(pred, a.n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled, recon=a.recon)
+        rec = evaluate_one(pred, n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled, recon=a.recon)
         records.append(rec)
         print(json.dumps(rec), flush=True)
     summary = summarize(records, ("warm", "pooled") if a.pooled else ("warm",))

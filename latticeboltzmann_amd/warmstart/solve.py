@@ -7,6 +7,7 @@ FP64 ``Lattice`` whose ``every`` steps between residual checks are replayed
 from one HIP graph.
 """
 
+import math
 import os
 import time
 
@@ -16,10 +17,20 @@ import torch
 from .. import _native as nat
 from .. import kernels
 from ..lattice import Lattice, StepGraph, omega_for_cavity
+from .controller import WarmStartUnavailable
 from .features import GRID, RE_RANGE, U_RANGE, features, in_channels
 from .network import Surrogate
 
-DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "surrogate_weights.npz")
+_HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(_HERE, "surrogate_weights.npz")
+# the network for the upper Reynolds range (trained on cavities with N of 512
+# and 1024, where BGK stays stable up to Re 5000); load_default() routes each
+# cavity to the network whose ranges cover it
+HIGH_RE_WEIGHTS = os.path.join(_HERE, "surrogate_weights_high_re.npz")
+
+
+class OutOfRange(WarmStartUnavailable, ValueError):
+    """(Re, U) outside the ranges the surrogate was trained on."""
 
 
 class Predictor:
@@ -57,9 +68,70 @@ class Predictor:
                       for re, u in params])
         return torch.as_tensor(x).to(self.net.device)
 
-    def predict(self, re, u):
-        """(m, m, 3) FP32 device tensor: ((rho - 1)/U^2, ux/U, uy/U) on cell centres."""
+    def covers(self, re, u):
+        """True when (Re, U) lies in the ranges the network was trained on."""
+        return (self.re_range[0] <= re <= self.re_range[1]
+                and self.u_range[0] <= u <= self.u_range[1])
+
+    def predict(self, re, u, extrapolate=False):
+        """(m, m, 3) FP32 device tensor: ((rho - 1)/U^2, ux/U, uy/U) on cell centres.
+
+        Raises ``OutOfRange`` (a ``WarmStartUnavailable``, so the controller
+        cold-starts) for (Re, U) outside the trained ranges, unless
+        ``extrapolate``: the inputs are normalized against those ranges, and
+        what the network answers outside them was never fitted."""
+        if not extrapolate and not self.covers(re, u):
+            raise OutOfRange("Re %.6g, U %.6g outside the trained ranges Re %s, U %s"
+                             % (re, u, self.re_range, self.u_range))
         return self.net.forward(self.inputs([(re, u)]))[0].contiguous()
+
+
+class PredictorSet:
+    """Several trained surrogates, each for its own (Re, U) ranges: a cavity
+    goes to the first member whose ranges cover it, and a cavity no member
+    covers raises ``OutOfRange`` (the controller then cold-starts).  Members
+    share the cell grid."""
+
+    def __init__(self, members):
+        self.members = list(members)
+        if not self.members:
+            raise ValueError("a predictor set needs at least one member")
+        grids = {m.grid for m in self.members}
+        if len(grids) != 1:
+            raise ValueError("members must share one cell grid, got %s" % sorted(grids))
+        self.grid = self.members[0].grid
+        self.pool = self.members[0].pool
+        self.re_range = (min(m.re_range[0] for m in self.members),
+                         max(m.re_range[1] for m in self.members))
+        self.u_range = (min(m.u_range[0] for m in self.members),
+                        max(m.u_range[1] for m in self.members))
+
+    def member(self, re, u):
+        for m in self.members:
+            if m.covers(re, u):
+                return m
+        return None
+
+    def covers(self, re, u):
+        return self.member(re, u) is not None
+
+    def predict(self, re, u, extrapolate=False):
+        m = self.member(re, u)
+        if m is None:
+            if not extrapolate:
+                raise OutOfRange("Re %.6g, U %.6g outside every member's ranges" % (re, u))
+            m = min(self.members, key=lambda p: abs(math.log(re) - math.log(
+                min(max(re, p.re_range[0]), p.re_range[1]))))
+        return m.predict(re, u, extrapolate=extrapolate)
+
+
+def load_default(device=None):
+    """The shipped networks: a ``PredictorSet`` of the low- and high-Reynolds
+    networks when both are present, else the single low-Reynolds ``Predictor``."""
+    low = Predictor.load(DEFAULT_WEIGHTS, device)
+    if not os.path.exists(HIGH_RE_WEIGHTS):
+        return low
+    return PredictorSet([low, Predictor.load(HIGH_RE_WEIGHTS, device)])
 
 
 RECONSTRUCTIONS = ("bilinear", "walls", "cubic")
@@ -121,7 +193,9 @@ class CavityStepper:
         if self.predictor is None:
             raise RuntimeError("a warm start needs a predictor")
         q = self.predictor.predict(self.re, self.u)
-        pool = getattr(self.predictor, "pool", 2)
+        # the prediction holds averages over cells of n/m x n/m nodes
+        m = q.shape[0]
+        pool = self.n // m if self.n % m == 0 else getattr(self.predictor, "pool", 2)
         self.lat.load(warm_populations(q, self.n, self.n, self.u, self.omega, self.neq, self.recon, pool))
         self.lat.time = 0
 

@@ -62,6 +62,40 @@ def test_forced_stall_falls_back_to_cold():
     assert r["steps"] == r["warm_steps"] + r["cold_steps"]
 
 
+def test_unavailable_warm_start_runs_cold():
+    from latticeboltzmann_amd.warmstart.controller import WarmStartUnavailable
+
+    class NoPrediction(MockStepper):
+        def init_warm(self):
+            self.calls.append("warm")
+            raise WarmStartUnavailable("outside the trained range")
+
+    st = NoPrediction(geometric(1e-3, 0.5), geometric(1e-2, 0.9))
+    r = Controller(tol=1e-6).run(st, warm=True)
+    assert r["fell_back"] and r["reason"] == "no warm start"
+    assert st.calls == ["warm", "cold"] and r["warm_steps"] == 0
+    assert r["steps"] == r["cold_steps"] == 1000 * math.ceil(math.log(1e-4) / math.log(0.9))
+
+
+def test_predictor_refuses_inputs_outside_its_ranges():
+    from latticeboltzmann_amd.warmstart.controller import WarmStartUnavailable
+    from latticeboltzmann_amd.warmstart.solve import OutOfRange, Predictor
+
+    class Net:  # no forward pass: the range check comes first
+        in_ch = IN_CHANNELS
+
+        def forward(self, x):
+            raise AssertionError("forward must not run for out-of-range inputs")
+
+    p = Predictor(Net(), re_range=(100.0, 1000.0), u_range=(0.05, 0.1))
+    assert p.covers(100.0, 0.05) and p.covers(1000.0, 0.1) and p.covers(400.0, 0.07)
+    for re, u in [(99.0, 0.07), (1001.0, 0.07), (400.0, 0.049), (400.0, 0.11), (5000.0, 0.1)]:
+        assert not p.covers(re, u)
+        with pytest.raises(OutOfRange):
+            p.predict(re, u)
+    assert issubclass(OutOfRange, WarmStartUnavailable)
+
+
 def test_nan_falls_back_immediately():
     st = MockStepper(lambda k: float("nan"), geometric(1e-3, 0.5))
     r = Controller(tol=1e-6).run(st, warm=True)
@@ -149,11 +183,19 @@ def test_evaluation_summary():
     for i, (cs, ws, cst, wst, d, fb) in enumerate([(100, 40, 1.0, 0.40, 2e-5, False),
                                                      (200, 50, 2.0, 0.55, 5e-5, False),
                                                      (300, 150, 3.0, 1.60, 2e-4, True)]):
-        recs.append({"cold": {"steps": cs, "seconds": cst},
+        recs.append({"n": 256 * (1 + i % 2), "re": 100.0, "u": 0.1,
+                     "cold": {"steps": cs, "seconds": cst, "reason": "converged"},
                      "warm": {"steps": ws, "seconds": wst, "fell_back": fb, "diff_max": d,
                               "step_speedup": cs / ws, "time_speedup": cst / wst}})
-    w = summarize(recs)["warm"]
+    # a cavity whose cold start never converged has no yardstick: listed, not summarized
+    recs.append({"n": 1024, "re": 4900.0, "u": 0.05,
+                 "cold": {"steps": 4000000, "seconds": 40.0, "reason": "diverged"}})
+    out = summarize(recs)
+    w = out["warm"]
     assert w["median_step_speedup"] == 2.5 and w["fallbacks"] == 1
     assert np.isclose(w["total_time_speedup"], 6.0 / 2.55)
     assert np.isclose(w["min_time_speedup"], 3.0 / 1.6)
     assert w["diff_max_worst"] == 2e-4 and np.isclose(w["frac_diff_below_1e-4"], 2 / 3)
+    assert [c["n"] for c in out["cold_not_converged"]] == [1024]
+    assert out["by_n"]["warm_256"]["cavities"] == 2 and out["by_n"]["warm_512"]["cavities"] == 1
+    assert out["by_n"]["warm_512"]["median_time_speedup"] == 2.0 / 0.55
