@@ -32,7 +32,9 @@ committed summary under profiles/, null if absent), and
 ``cpu_baseline``: the reference's CPU paths restated by the oracle, timed on
 one host core on a bounded sample of the same 2048^2 cavity: ``value`` is the
 opt2 structure (numpy walls + the C++ _lbkernels collide restated in C), the
-fastest; the numpy opt1 restatement and a fused C step are in ``sample``.
+fastest; the numpy opt1 restatement and a fused C step are in ``sample``;
+``legs`` adds the 256^2 shear wave (1000 steps) and the 512^2 cavity crop
+(200 steps), every leg timed in full.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -171,12 +173,27 @@ def profiled_traffic(name, variant=None):
     return j.get("hbm_bytes_per_launch")
 
 
-def cpu_baseline(n, budget):
-    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), "--n", str(n),
-           "--budget", str(budget), "--max-steps", "40"]
-    out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=600).stdout
-    rec = json.loads(out.strip().splitlines()[-1])
-    return {
+def _cpu_leg(args):
+    cmd = [sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py")] + list(args)
+    out = subprocess.run(cmd, check=True, capture_output=True, text=True, timeout=900).stdout
+    return json.loads(out.strip().splitlines()[-1])
+
+
+def _leg_summary(rec):
+    keep = ("numpy_opt1", "opt2", "opt2_native", "c_port")
+    return {k: {"mlups": round(rec[k]["mlups"], 3), "steps": rec[k]["steps"],
+                "seconds": round(rec[k]["seconds"], 2)} for k in keep if rec.get(k)}
+
+
+def cpu_baseline(n, budget, workloads=True):
+    """The reference's CPU paths (restated by the oracle) on one host core.
+
+    ``value`` is the opt2 structure on the bench's own workload (the n x n
+    cavity, a bounded sample of its steps); with ``workloads`` the 256^2
+    shear wave (1000 steps) and the 512^2 cavity crop (200 steps) are timed
+    in full beside it, every leg of each."""
+    rec = _cpu_leg(["--n", str(n), "--budget", str(budget), "--max-steps", "40"])
+    out = {
         # the reference's fastest CPU path, restated: cavity_opt2.py's numpy
         # walls plus the C++ collide of _lbkernels (d2q9_colliden), one core
         "value": round(rec["opt2"]["mlups"], 4),
@@ -192,7 +209,13 @@ def cpu_baseline(n, budget):
                      "%.3f" % rec["opt2_native"]["mlups"] if rec.get("opt2_native") else "n/a",
                      rec["numpy_opt1"]["mlups"], rec["c_port"]["mlups"]),
         "numpy_opt1_mlups": round(rec["numpy_opt1"]["mlups"], 4),
+        "legs": {"cavity_%d" % n: _leg_summary(rec)},
     }
+    if workloads:
+        for name in ("shear256", "crop512"):
+            r = _cpu_leg(["--workload", name])
+            out["legs"][name] = dict(_leg_summary(r), workload=r["workload"])
+    return out
 
 
 def batch_sweep(steps=2000):
