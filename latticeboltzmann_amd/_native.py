@@ -170,6 +170,23 @@ def load(path=LIB_PATH):
     return lib
 
 
+_BUILD_ID = None
+
+
+def build_id(path=LIB_PATH):
+    """Content hash of the library file (16 hex digits): names the build the
+    autotuner's cached choices were measured with."""
+    global _BUILD_ID
+    if _BUILD_ID is None:
+        import hashlib
+        h = hashlib.sha1()
+        with open(path, "rb") as fh:
+            for chunk in iter(lambda: fh.read(1 << 20), b""):
+                h.update(chunk)
+        _BUILD_ID = h.hexdigest()[:16]
+    return _BUILD_ID
+
+
 def check(status, what):
     """Raise for a non-zero status returned by a C-ABI call."""
     if status == LBM_OK:
