@@ -47,6 +47,16 @@ __device__ __forceinline__ uint16_t f2bf(float x) { return __builtin_bit_cast(ui
 constexpr int TM = 64, TN = 64, TK = 32, TP = TK + 8;
 constexpr int LOAD_SCALAR = 0, LOAD_K = 1, LOAD_ROW = 2;
 
+// A convolution operand gathered on the fly instead of read from a
+// materialised im2col matrix: element (pixel p, index q) with q = tap * C + c,
+// tap = (dy + 1) * 3 + (dx + 1), is X[pixel p + sign * (dy, dx)][c] of the NHWC
+// BF16 activation X (H x W images, C channels), 0 outside the image.  sign +1
+// is the im2col of a 3x3 same convolution; sign -1 gathers, for input pixel
+// p, the output pixels that read p (the data gradient).  sign 0: no gather.
+struct Gather {
+    int H, W, C, sign;
+};
+
 struct GemmArgs {
     const uint16_t *A;
     int64_t sam, sak;
@@ -59,7 +69,68 @@ struct GemmArgs {
     int M, N, K, kchunk, accumulate, relu;
     float *work;   // split-K partials
     int mode_a, mode_b;
+    Gather ga;     // A(m, k) gathered with pixel m, index k (ga.sign != 0)
+    Gather gb;     // B(k, n) gathered with pixel k, index n (gb.sign != 0)
 };
+
+// Index into X of gathered element (pixel p, index q), and whether it lies in
+// the image.
+__device__ __forceinline__ int64_t gather_index(const Gather &G, int p, int q, bool &inside) {
+    const int tap = q / G.C, c = q - tap * G.C;
+    const int dy = (tap / 3 - 1) * G.sign, dx = (tap % 3 - 1) * G.sign;
+    const int x = p % G.W, y = (p / G.W) % G.H;
+    inside = unsigned(y + dy) < unsigned(G.H) && unsigned(x + dx) < unsigned(G.W);
+    return (int64_t(p) + int64_t(dy) * G.W + dx) * G.C + c;
+}
+
+__device__ __forceinline__ uint16_t gather_one(const uint16_t *__restrict__ X, const Gather &G,
+                                               int p, int q) {
+    bool in;
+    const int64_t i = gather_index(G, p, q, in);
+    return in ? X[i] : uint16_t(0);
+}
+
+// tile[row][k] <- X(row0 + row, k0 + k) for a gathered operand, zero outside
+// rows < nrows, k < kend.  pix_row: the row is the pixel and k the (tap,
+// channel) index (A of the forward and data-gradient GEMMs); otherwise k is
+// the pixel and the row the index (B of the weight-gradient GEMM).  With C a
+// multiple of 8, the 8 indices a thread loads lie in one tap: one 16-byte load.
+__device__ __forceinline__ void load_tile_gather(uint16_t (*tile)[TP], const uint16_t *__restrict__ X,
+                                                 const Gather &G, int row0, int nrows, int k0,
+                                                 int kend, bool pix_row, int t) {
+    const bool vec = G.C % 8 == 0;
+    if (pix_row) {
+        const int rr = t >> 2, kc = (t & 3) * 8;
+        const int r = row0 + rr, k = k0 + kc;
+        if (vec && r < nrows && k + 8 <= kend) {
+            bool in;
+            const int64_t i = gather_index(G, r, k, in);
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (in) v = *reinterpret_cast<const uint4 *>(X + i);
+            *reinterpret_cast<uint4 *>(&tile[rr][kc]) = v;
+            return;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            tile[rr][kc + i] = (r < nrows && k + i < kend) ? gather_one(X, G, r, k + i) : uint16_t(0);
+        return;
+    }
+    const int kr = t >> 3, rc = (t & 7) * 8;
+    const int k = k0 + kr, r = row0 + rc;
+    if (vec && k < kend && r + 8 <= nrows) {
+        bool in;
+        const int64_t i = gather_index(G, k, r, in);
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (in) v = *reinterpret_cast<const uint4 *>(X + i);
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&v);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) tile[rc + i][kr] = e[i];
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        tile[rc + i][kr] = (k < kend && r + i < nrows) ? gather_one(X, G, k, r + i) : uint16_t(0);
+}
 
 // tile[row][k] <- X(row0 + row, k0 + k) for row < 64, k < 32; X(r, k) =
 // X[r*s_row + k*s_k], zero outside rows < nrows, k < kend.
@@ -108,8 +179,10 @@ __global__ __launch_bounds__(256) void k_gemm(GemmArgs g) {
         for (int j = 0; j < 2; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        load_tile(As, g.A, g.sam, g.sak, m0, g.M, k0, kend, g.mode_a, t);
-        load_tile(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, kend, g.mode_b, t);
+        if (g.ga.sign) load_tile_gather(As, g.A, g.ga, m0, g.M, k0, kend, true, t);
+        else load_tile(As, g.A, g.sam, g.sak, m0, g.M, k0, kend, g.mode_a, t);
+        if (g.gb.sign) load_tile_gather(Bs, g.B, g.gb, n0, g.N, k0, kend, false, t);
+        else load_tile(Bs, g.B, g.sbn, g.sbk, n0, g.N, k0, kend, g.mode_b, t);
         __syncthreads();
         bf16x8 a[2], b[2];
 #pragma unroll
@@ -252,13 +325,26 @@ __global__ __launch_bounds__(256) void k_bias_grad_partial(const float *__restri
     }
 }
 
+// Pass 2: block n adds the nblocks (<= 256) partials of column n, one per
+// thread, in a fixed-shape tree, so all of them are loaded at once instead of
+// one after another by a single thread.
 __global__ __launch_bounds__(256) void k_bias_grad_final(const double *__restrict__ partial, int N,
                                                          int nblocks, float *__restrict__ db) {
-    const int n = blockIdx.x * 256 + threadIdx.x;
-    if (n >= N) return;
-    double v = 0.0;
-    for (int b = 0; b < nblocks; ++b) v += partial[int64_t(b) * N + n];
-    db[n] = float(v);
+    const int n = blockIdx.x, t = threadIdx.x;
+    const double v = t < nblocks ? partial[int64_t(t) * N + n] : 0.0;
+    const double tot = block_sum(v);
+    if (t == 0) db[n] = float(tot);
+}
+
+// Weights W[cout][tap][cin] -> Wf[tap][cout][cin] (BF16): the B operand of the
+// data-gradient GEMM, dX[p][c] = sum over (tap, co) of dY[p - tap][co] Wf[tap][co][c].
+__global__ __launch_bounds__(256) void k_flip_weights(const uint16_t *__restrict__ w, int cin,
+                                                      int cout, uint16_t *__restrict__ wf) {
+    const int i = blockIdx.x * 256 + threadIdx.x;  // over tap, cout, cin
+    if (i >= 9 * cout * cin) return;
+    const int c = i % cin, r = i / cin;
+    const int co = r % cout, tap = r / cout;
+    wf[i] = w[(int64_t(co) * 9 + tap) * cin + c];
 }
 
 // 2x2 average pool (NHWC): y [B][H/2][W/2][C].
@@ -602,6 +688,16 @@ static int load_mode(const void *X, int64_t s_row, int64_t s_k) {
     return LOAD_SCALAR;
 }
 
+// Launch the GEMM (and the ordered split-K reduction when splits > 1).
+static int run_gemm(const GemmArgs &g, int splits, hipStream_t s) {
+    dim3 grid((g.N + TN - 1) / TN, (g.M + TM - 1) / TM, splits);
+    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
+    if (splits > 1)
+        hipLaunchKernelGGL(k_gemm_reduce, dim3(blocks_for(int64_t(g.M) * g.N)), dim3(256), 0, s, g,
+                           splits);
+    return launch_status();
+}
+
 extern "C" {
 
 int64_t lbm_nn_gemm_workspace(int M, int N, int ksplit) {
@@ -620,13 +716,66 @@ int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t 
     const int splits = (K + kchunk - 1) / kchunk;
     GemmArgs g{static_cast<const uint16_t *>(A), sam, sak, static_cast<const uint16_t *>(B), sbk, sbn,
                C, static_cast<uint16_t *>(Cb), ldc, bias, M, N, K, kchunk, accumulate, relu, work,
-               load_mode(A, sam, sak), load_mode(B, sbn, sbk)};
-    hipStream_t s = as_stream(stream);
-    dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
-    hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, s, g);
-    if (splits > 1)
-        hipLaunchKernelGGL(k_gemm_reduce, dim3(blocks_for(int64_t(M) * N)), dim3(256), 0, s, g, splits);
+               load_mode(A, sam, sak), load_mode(B, sbn, sbk), Gather{1, 1, 1, 0},
+               Gather{1, 1, 1, 0}};
+    return run_gemm(g, splits, as_stream(stream));
+}
+
+int lbm_nn_conv3x3(const void *x, int B, int H, int W, int cin, const void *w, int cout,
+                   const float *bias, float *y, void *yb, int relu, void *stream) {
+    if (!x || !w || (!y && !yb)) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || cin <= 0 || cout <= 0) return LBM_E_SIZE;
+    const int64_t P = int64_t(B) * H * W;
+    if (P * 9 * cin >= (int64_t(1) << 31) || P * cout >= (int64_t(1) << 31)) return LBM_E_SIZE;
+    if (cin % 8 == 0 && reinterpret_cast<uintptr_t>(x) % 16) return LBM_E_ARG;
+    const int K = 9 * cin;
+    GemmArgs g{static_cast<const uint16_t *>(x), 0, 0, static_cast<const uint16_t *>(w), 1, K, y,
+               static_cast<uint16_t *>(yb), cout, bias, int(P), cout, K, (K + TK - 1) / TK * TK, 0,
+               relu, nullptr, LOAD_SCALAR, load_mode(w, K, 1), Gather{H, W, cin, 1},
+               Gather{1, 1, 1, 0}};
+    return run_gemm(g, 1, as_stream(stream));
+}
+
+int lbm_nn_conv3x3_wgrad(const void *x, const void *dy, int B, int H, int W, int cin, int cout,
+                         float *dw, int ksplit, float *work, void *stream) {
+    if (!x || !dy || !dw) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || cin <= 0 || cout <= 0 || ksplit <= 0) return LBM_E_SIZE;
+    const int64_t P = int64_t(B) * H * W;
+    if (P * 9 * cin >= (int64_t(1) << 31) || P * cout >= (int64_t(1) << 31)) return LBM_E_SIZE;
+    if (cin % 8 == 0 && reinterpret_cast<uintptr_t>(x) % 16) return LBM_E_ARG;
+    if (ksplit > 1 && !work) return LBM_E_WORKSPACE;
+    const int K = int(P), N = 9 * cin;
+    int kchunk = (K + ksplit - 1) / ksplit;
+    kchunk = (kchunk + TK - 1) / TK * TK;
+    const int splits = (K + kchunk - 1) / kchunk;
+    GemmArgs g{static_cast<const uint16_t *>(dy), 1, cout, static_cast<const uint16_t *>(x), 0, 0,
+               dw, nullptr, N, nullptr, cout, N, K, kchunk, 0, 0, work, load_mode(dy, 1, cout),
+               LOAD_SCALAR, Gather{1, 1, 1, 0}, Gather{H, W, cin, 1}};
+    return run_gemm(g, splits, as_stream(stream));
+}
+
+int lbm_nn_conv3x3_flip_weights(const void *w, int cin, int cout, void *wf, void *stream) {
+    if (!w || !wf) return LBM_E_NULL;
+    if (cin <= 0 || cout <= 0) return LBM_E_SIZE;
+    hipLaunchKernelGGL(k_flip_weights, dim3(blocks_for(int64_t(9) * cout * cin)), dim3(256), 0,
+                       as_stream(stream), static_cast<const uint16_t *>(w), cin, cout,
+                       static_cast<uint16_t *>(wf));
     return launch_status();
+}
+
+int lbm_nn_conv3x3_dgrad(const void *dy, const void *wf, int B, int H, int W, int cin, int cout,
+                         float *dx, int accumulate, void *stream) {
+    if (!dy || !wf || !dx) return LBM_E_NULL;
+    if (B <= 0 || H <= 0 || W <= 0 || cin <= 0 || cout <= 0) return LBM_E_SIZE;
+    const int64_t P = int64_t(B) * H * W;
+    if (P * 9 * cout >= (int64_t(1) << 31) || P * cin >= (int64_t(1) << 31)) return LBM_E_SIZE;
+    if (cout % 8 == 0 && reinterpret_cast<uintptr_t>(dy) % 16) return LBM_E_ARG;
+    const int K = 9 * cout;
+    GemmArgs g{static_cast<const uint16_t *>(dy), 0, 0, static_cast<const uint16_t *>(wf), cin, 1,
+               dx, nullptr, cin, nullptr, int(P), cin, K, (K + TK - 1) / TK * TK, accumulate, 0,
+               nullptr, LOAD_SCALAR, load_mode(wf, 1, cin), Gather{H, W, cout, -1},
+               Gather{1, 1, 1, 0}};
+    return run_gemm(g, 1, as_stream(stream));
 }
 
 int lbm_nn_im2col3x3(const void *x, int B, int H, int W, int C, void *cols, void *stream) {
@@ -675,7 +824,7 @@ int lbm_nn_bias_grad(const float *dy, int64_t M, int N, float *db, double *work,
     const int nb = int((M + rows - 1) / rows);
     for (int c0 = 0; c0 < N; c0 += 256)
         hipLaunchKernelGGL(k_bias_grad_partial, dim3(nb), dim3(256), 0, s, dy, M, N, c0, rows, work);
-    hipLaunchKernelGGL(k_bias_grad_final, dim3(blocks_for(N)), dim3(256), 0, s, work, N, nb, db);
+    hipLaunchKernelGGL(k_bias_grad_final, dim3(N), dim3(256), 0, s, work, N, nb, db);
     return launch_status();
 }
 

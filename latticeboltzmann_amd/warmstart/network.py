@@ -1,9 +1,11 @@
 """Convolutional encoder-decoder of the warm start, on the project's HIP kernels.
 
 Every arithmetic step runs in ``liblbm_d2q9.so`` (``lbm_nn_*``): 3x3 same
-convolutions as im2col + BF16 MFMA GEMM with FP32 accumulation and a fused
-bias/ReLU epilogue, 2x2 average pooling, nearest 2x upsampling, channel
-concatenation, their backward passes, the MSE loss and Adam.  torch provides
+convolutions as BF16 MFMA GEMMs with FP32 accumulation whose tile loader
+gathers the shifted activations itself (no im2col matrix in memory: forward,
+weight gradient and data gradient), a fused bias/ReLU epilogue, 2x2 average
+pooling, nearest 2x upsampling, channel concatenation, their backward passes,
+the MSE loss and Adam.  torch provides
 device memory only (no autograd, no torch arithmetic on the hot path).
 
 Layout: activations NHWC (FP32, plus a BF16 copy that feeds the next GEMM);
@@ -114,15 +116,11 @@ class Surrogate:
         cin_, cout, relu_ = self.spec[name]
         assert cin == cin_, (name, cin, cin_)
         relu = relu_ if relu is None else relu
-        P, K = B * H * W, 9 * cin
-        s = nat.stream_ptr()
-        cols = torch.empty((P, K), dtype=torch.bfloat16, device=self.device)
-        nat.call("lbm_nn_im2col3x3", _p(xb), B, H, W, cin, _p(cols), s)
         y = torch.empty((B, H, W, cout), dtype=torch.float32, device=self.device)
         yb = torch.empty((B, H, W, cout), dtype=torch.bfloat16, device=self.device)
         wb, bias = self.views(name, self.pb)[0], self.views(name)[1]
-        nat.call("lbm_nn_gemm", _p(cols), K, 1, _p(wb), 1, K, _p(y), _p(yb), cout, _p(bias),
-                 P, cout, K, 0, int(relu), 1, None, s)
+        nat.call("lbm_nn_conv3x3", _p(xb), B, H, W, cin, _p(wb), cout, _p(bias), _p(y), _p(yb),
+                 int(relu), nat.stream_ptr())
         return y, yb
 
     def _pool(self, x):
@@ -193,29 +191,26 @@ class Surrogate:
             nat.call("lbm_nn_relu_backward", _p(y), _p(dy), _p(dyb), dy.numel(), s)
         else:
             nat.call("lbm_nn_to_bf16", _p(dy), _p(dyb), dy.numel(), s)
-        cols = torch.empty((P, K), dtype=torch.bfloat16, device=self.device)
-        nat.call("lbm_nn_im2col3x3", _p(xb), B, H, W, cin, _p(cols), s)
         gw, gbias = self.views(name, self.g)
         ks = self._ksplit(cout, K, P)
         work = None
         if ks > 1:
             work = torch.empty(int(nat.load().lbm_nn_gemm_workspace(cout, K, ks)) // 4,
                                dtype=torch.float32, device=self.device)
-        # dW[co][k] = sum_p dy[p][co] cols[p][k]
-        nat.call("lbm_nn_gemm", _p(dyb), 1, cout, _p(cols), K, 1, _p(gw), None, K, None,
-                 cout, K, P, 0, 0, ks, None if work is None else _p(work), s)
+        # dW[co][k] = sum_p dy[p][co] x(p, k), x(p, k) the gathered 3x3 neighbourhood
+        nat.call("lbm_nn_conv3x3_wgrad", _p(xb), _p(dyb), B, H, W, cin, cout, _p(gw), ks,
+                 None if work is None else _p(work), s)
         bw = torch.empty(int(nat.load().lbm_nn_bias_grad_workspace(cout)) // 8, dtype=torch.float64,
                          device=self.device)
         nat.call("lbm_nn_bias_grad", _p(dy), P, cout, _p(gbias), _p(bw), s)
         if not need_dx:
             return None
-        # dcols[p][k] = sum_co dy[p][co] W[co][k]; dx = col2im(dcols)
+        # dx[p][c] = sum over taps and co of dy[p - tap][co] W[co][tap][c]
         wb = self.views(name, self.pb)[0]
-        dcols = torch.empty((P, K), dtype=torch.float32, device=self.device)
-        nat.call("lbm_nn_gemm", _p(dyb), cout, 1, _p(wb), K, 1, _p(dcols), None, K, None,
-                 P, K, cout, 0, 0, 1, None, s)
+        wf = torch.empty(9 * cin * cout, dtype=torch.bfloat16, device=self.device)
+        nat.call("lbm_nn_conv3x3_flip_weights", _p(wb), cin, cout, _p(wf), s)
         dx = torch.empty((B, H, W, cin), dtype=torch.float32, device=self.device)
-        nat.call("lbm_nn_col2im3x3", _p(dcols), B, H, W, cin, _p(dx), 0, s)
+        nat.call("lbm_nn_conv3x3_dgrad", _p(dyb), _p(wf), B, H, W, cin, cout, _p(dx), 0, s)
         return dx
 
     def backward(self, cache, dout, need_input_grad=False):

@@ -120,6 +120,60 @@ def test_im2col_col2im(gpu, C):
     np.testing.assert_allclose(dx.double().cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
 
 
+@pytest.mark.parametrize("cin,cout,ksplit", [(3, 16, 1), (16, 3, 3), (16, 32, 4), (6, 8, 1)])
+def test_conv3x3_implicit_gemm(gpu, cin, cout, ksplit):
+    # the gathering convolution GEMMs against im2col + GEMM: forward and weight
+    # gradient load the same tiles in the same k order (bitwise equal), the data
+    # gradient sums the nine taps inside the GEMM instead of in col2im
+    nat = _nat()
+    rng = np.random.default_rng(16)
+    B, H, W = 2, 9, 7
+    P, K, sp = B * H * W, 9 * cin, nat.stream_ptr()
+    x, _ = _bf(rng, (B, H, W, cin))
+    w, wn = _bf(rng, (cout, 3, 3, cin), 0.2)
+    dy, _ = _bf(rng, (B, H, W, cout))
+    bias = torch.as_tensor(rng.normal(size=cout).astype(np.float32)).cuda()
+    cols = torch.empty((P, K), dtype=torch.bfloat16, device="cuda")
+    nat.call("lbm_nn_im2col3x3", _p(x), B, H, W, cin, _p(cols), sp)
+    # forward with bias and ReLU, both outputs
+    ye, yeb = (torch.empty((P, cout), dtype=t, device="cuda") for t in (torch.float32, torch.bfloat16))
+    nat.call("lbm_nn_gemm", _p(cols), K, 1, _p(w), 1, K, _p(ye), _p(yeb), cout, _p(bias),
+             P, cout, K, 0, 1, 1, None, sp)
+    y, yb = torch.empty_like(ye), torch.empty_like(yeb)
+    nat.call("lbm_nn_conv3x3", _p(x), B, H, W, cin, _p(w), cout, _p(bias), _p(y), _p(yb), 1, sp)
+    assert torch.equal(y, ye) and torch.equal(yb, yeb)
+    # weight gradient, split over pixels
+    lib = nat.load()
+    work = torch.empty(max(1, int(lib.lbm_nn_gemm_workspace(cout, K, ksplit)) // 4),
+                       dtype=torch.float32, device="cuda")
+    dwe, dw = (torch.empty((cout, K), dtype=torch.float32, device="cuda") for _ in range(2))
+    nat.call("lbm_nn_gemm", _p(dy), 1, cout, _p(cols), K, 1, _p(dwe), None, K, None,
+             cout, K, P, 0, 0, ksplit, _p(work), sp)
+    nat.call("lbm_nn_conv3x3_wgrad", _p(x), _p(dy), B, H, W, cin, cout, _p(dw), ksplit, _p(work), sp)
+    assert torch.equal(dw, dwe)
+    # data gradient: against GEMM + col2im, and against float64 on the CPU
+    dcols = torch.empty((P, K), dtype=torch.float32, device="cuda")
+    nat.call("lbm_nn_gemm", _p(dy), cout, 1, _p(w), K, 1, _p(dcols), None, K, None,
+             P, K, cout, 0, 0, 1, None, sp)
+    dxe = torch.empty((B, H, W, cin), dtype=torch.float32, device="cuda")
+    nat.call("lbm_nn_col2im3x3", _p(dcols), B, H, W, cin, _p(dxe), 0, sp)
+    wf = torch.empty(9 * cin * cout, dtype=torch.bfloat16, device="cuda")
+    nat.call("lbm_nn_conv3x3_flip_weights", _p(w), cin, cout, _p(wf), sp)
+    assert np.array_equal(wf.view(3, 3, cout, cin).double().cpu().numpy(), wn.transpose(1, 2, 0, 3))
+    dx = torch.full((B, H, W, cin), 0.25, dtype=torch.float32, device="cuda")
+    nat.call("lbm_nn_conv3x3_dgrad", _p(dy), _p(wf), B, H, W, cin, cout, _p(dx), 1, sp)
+    ref = 0.25 + s.col2im(dy.double().cpu().numpy().reshape(P, cout) @ wn.reshape(cout, K),
+                          (B, H, W, cin))
+    np.testing.assert_allclose(dx.double().cpu().numpy(), ref, rtol=1e-5, atol=1e-5)
+    np.testing.assert_allclose(dx.cpu().numpy() - 0.25, dxe.cpu().numpy(), rtol=1e-5, atol=1e-5)
+    # argument checks
+    assert lib.lbm_nn_conv3x3(None, B, H, W, cin, _p(w), cout, None, _p(y), None, 0, sp) == -1
+    assert lib.lbm_nn_conv3x3(_p(x), 0, H, W, cin, _p(w), cout, None, _p(y), None, 0, sp) == -2
+    assert lib.lbm_nn_conv3x3_wgrad(_p(x), _p(dy), B, H, W, cin, cout, _p(dw), 2, None, sp) == -5
+    assert lib.lbm_nn_conv3x3_dgrad(_p(dy), _p(wf), B, H, W, cin, -1, _p(dx), 0, sp) == -2
+    torch.cuda.synchronize()
+
+
 def test_pool_upsample_concat_kernels(gpu):
     nat = _nat()
     rng = np.random.default_rng(7)
