@@ -87,15 +87,18 @@ int lbm_moments_f32(const float *f, int64_t nx, int64_t ny, int layout, float *r
 int lbm_stream_periodic(const void *src, void *dst, int64_t nx, int64_t ny,
                         int elem_bytes, void *stream);
 
-/* K full time steps src -> dst in one launch (temporal blocking, K = 2..4
- * fixed by the variant): each block computes the first K-1 steps of its tile
- * and a shrinking halo in LDS and writes only the last, so HBM sees about one
- * read and one write of each population per K steps.  Bitwise equal to K
+/* K full time steps src -> dst in one launch (temporal blocking, K = 2..6
+ * fixed by the variant).  Tile variants: each block computes the first K-1
+ * steps of its tile and a shrinking halo in LDS and writes only the last.
+ * Sweep variants: each wave walks a strip of 64 columns along x and keeps the
+ * intermediate steps of the last rows in registers.  Either way HBM sees
+ * about one read and one write of each population per K steps.  Bitwise equal to K
  * lbm_step_layout_* calls (same arithmetic, cavity_opt1.py:341-343 K times);
  * rho/ux/uy (nullable, all or none) receive the pre-collision moments of the
  * last step.  variant picks (K, tile shape), 0 .. lbm_step_multi_variant_count()-1
- * or -1 for the default; lbm_step_multi_info gives its K and the smallest grid
- * it takes (LBM_E_SIZE for smaller grids).                                   */
+ * or -1 for the default of the grid size (lbm_step_multi_default_for);
+ * lbm_step_multi_info gives its K and the smallest grid it takes (LBM_E_SIZE
+ * for smaller grids).                                                        */
 int lbm_step_multi_f64(const double *src, double *dst, int64_t nx, int64_t ny, int layout,
                        double omega, int bc, double u0, double *rho, double *ux, double *uy,
                        int variant, void *stream);
@@ -103,7 +106,11 @@ int lbm_step_multi_f32(const float *src, float *dst, int64_t nx, int64_t ny, int
                        float omega, int bc, float u0, float *rho, float *ux, float *uy,
                        int variant, void *stream);
 int lbm_step_multi_variant_count(void);
-int lbm_step_multi_default_variant(void); /* the variant that -1 selects */
+int lbm_step_multi_default_variant(void); /* the tile default (below 4096^2 nodes) */
+/* The variant that -1 selects on an nx x ny grid: the default tile below
+ * 4096^2 nodes, the five-step register sweep from there (LBM_E_SIZE for an
+ * invalid grid).                                                             */
+int lbm_step_multi_default_for(int64_t nx, int64_t ny);
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 
 /* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with
@@ -217,10 +224,11 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
 #define LBM_COMPUTE_F64 1
 /* One fused step src -> dst on stored deviations; ux/uy (nullable, FP32
  * [k][l]) receive the pre-collision velocity.  variant in
- * 0 .. lbm_dev_variant_count()-1 or -1.  omega, u0 and scale are FP32 values;
- * the FP64 collide uses them exactly as given.                               */
+ * 0 .. lbm_dev_variant_count()-1 or -1.  omega, u0 and scale are taken in
+ * the compute type: the FP64 collide uses them as given, the FP32 collide
+ * rounds them to FP32.                                                       */
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                 float omega, int bc, float u0, float scale, int compute, int rounding,
+                 double omega, int bc, double u0, double scale, int compute, int rounding,
                  uint32_t seed, float *ux, float *uy, int variant, void *stream);
 /* K steps src -> dst in one launch (temporal blocking, K = 2..4 fixed by the
  * variant), as lbm_step_multi_* for stored deviations.  round_each = 1: LDS
@@ -233,7 +241,7 @@ int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, in
  * whose tile would not fit the 160 KB of LDS).  ux/uy receive the last step's
  * velocity.  lbm_dev_multi_info gives the variant's K and smallest grid.      */
 int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                       float omega, int bc, float u0, float scale, int compute, int rounding,
+                       double omega, int bc, double u0, double scale, int compute, int rounding,
                        uint32_t seed, uint32_t seed_stride, int round_each, float *ux, float *uy,
                        int variant, void *stream);
 int lbm_dev_multi_variant_count(void);
@@ -258,10 +266,10 @@ int lbm_dev_velocity_residual(const void *d, int64_t nx, int64_t ny, int layout,
 typedef struct lbm_batch_inst {
     int64_t offset; /* element offset of the instance's 9*nx*ny values in the pool */
     int32_t nx, ny;
-    float omega;
-    float u0;    /* lid velocity when bc == LBM_BC_CAVITY */
-    int32_t bc;  /* LBM_BC_PERIODIC or LBM_BC_CAVITY */
-    int32_t dir; /* observable: 0 = amplitude of u_x along y, 1 = of u_y along x */
+    double omega;   /* in the compute type, as lbm_dev_step */
+    double u0;      /* lid velocity when bc == LBM_BC_CAVITY */
+    int32_t bc;     /* LBM_BC_PERIODIC or LBM_BC_CAVITY */
+    int32_t dir;    /* observable: 0 = amplitude of u_x along y, 1 = of u_y along x */
 } lbm_batch_inst;
 
 typedef struct lbm_batch_tile {
@@ -280,6 +288,24 @@ int lbm_batch_step(const void *src, void *dst, int fmt, float scale,
                    const lbm_batch_inst *insts, const lbm_batch_tile *tiles, int ntiles,
                    int compute, int rounding, uint32_t seed, const uint32_t *epoch, int variant,
                    int blocks, void *stream);
+/* K steps (K = 2 or 4) of the instances of a work list in one launch, by a
+ * sweep along x that keeps the intermediate steps in registers: item i is
+ * worked by one wave of 64 lanes, which writes the rows [row0, row0 + rows)
+ * of strip `strip` of instance `inst` (strip s covers columns
+ * s (64 - 2(K-1)) .. + 64 - 2(K-1); an instance of 64 columns is one strip;
+ * lbm_batch_sweep_strips gives the count).  Every step is rounded as
+ * lbm_batch_step rounds it, step j (0 .. K-1) with seed + j * seed_stride
+ * combined with *epoch, so the launch is bitwise K lbm_batch_step launches.
+ * The items must cover each instance's rows and strips exactly once; every
+ * instance needs ny >= 64 and nx >= 2K.                                      */
+typedef struct lbm_batch_item {
+    int32_t inst, strip, row0, rows;
+} lbm_batch_item;
+int lbm_batch_sweep_strips(int steps, int ny);
+int lbm_batch_step_multi(const void *src, void *dst, int fmt, float scale,
+                         const lbm_batch_inst *insts, const lbm_batch_item *items, int nitems,
+                         int steps, int compute, int rounding, uint32_t seed, uint32_t seed_stride,
+                         const uint32_t *epoch, void *stream);
 /* Shear-wave amplitude of every instance by projection on its initial sine,
  * A = 2/(nx ny) sum u_dir sin(2 pi pos / n), FP64; written to
  * amp[*epoch * ninst + i] when *epoch < nslots.  tiles: 256-node tiles in
@@ -322,6 +348,23 @@ int64_t lbm_nn_gemm_workspace(int M, int N, int ksplit);
 int lbm_nn_gemm(const void *A, int64_t sam, int64_t sak, const void *B, int64_t sbk, int64_t sbn,
                 float *C, void *Cb, int64_t ldc, const float *bias, int M, int N, int K,
                 int accumulate, int relu, int ksplit, float *work, void *stream);
+/* 3x3 same convolutions without a materialised im2col matrix: the GEMM's
+ * tile loader gathers the shifted activations itself (16-byte loads when the
+ * gathered channel count is a multiple of 8).  x, dy BF16 NHWC; w BF16
+ * [cout][9*cin] ((dy*3 + dx)*cin + c); the sums run in FP32 over the same k
+ * order as im2col + lbm_nn_gemm, so the forward and weight gradient are
+ * bitwise those of the materialised form.
+ *   conv3x3:  y[p][co] = relu?(sum_k x(p, k) w[co][k] + bias[co])  (y FP32 and/or yb BF16)
+ *   wgrad:    dw[co][k] = sum_p dy[p][co] x(p, k)   (split-K over pixels as lbm_nn_gemm)
+ *   dgrad:    dx[p][c] (+)= sum_{tap, co} dy[p - tap][co] wf[tap][co][c], wf from
+ *             flip_weights (wf[tap][co][c] = w[co][tap*cin + c]).               */
+int lbm_nn_conv3x3(const void *x, int B, int H, int W, int cin, const void *w, int cout,
+                   const float *bias, float *y, void *yb, int relu, void *stream);
+int lbm_nn_conv3x3_wgrad(const void *x, const void *dy, int B, int H, int W, int cin, int cout,
+                         float *dw, int ksplit, float *work, void *stream);
+int lbm_nn_conv3x3_flip_weights(const void *w, int cin, int cout, void *wf, void *stream);
+int lbm_nn_conv3x3_dgrad(const void *dy, const void *wf, int B, int H, int W, int cin, int cout,
+                         float *dx, int accumulate, void *stream);
 /* cols[((b*H + y)*W + x)*9*C + (dy*3 + dx)*C + c] = x[b][y+dy-1][x+dx-1][c] (0 outside). */
 int lbm_nn_im2col3x3(const void *x, int B, int H, int W, int C, void *cols, void *stream);
 /* Adjoint of im2col3x3 (a gather, FP32): dx (+)= col2im(dcols).              */

@@ -72,6 +72,7 @@ SIGNATURES = {
                                   _i32, _vp]),
     "lbm_step_multi_variant_count": (_i32, []),
     "lbm_step_multi_default_variant": (_i32, []),
+    "lbm_step_multi_default_for": (_i32, [_i64, _i64]),
     "lbm_step_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
                                    ctypes.POINTER(_i64)]),
     "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),
@@ -94,9 +95,9 @@ SIGNATURES = {
     "lbm_dev_variant_count": (_i32, []),
     "lbm_dev_encode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
     "lbm_dev_decode": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _f64, _vp]),
-    "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32, _i32,
+    "lbm_dev_step": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f64, _i32, _f64, _f64, _i32,
                             _i32, ctypes.c_uint32, _vp, _vp, _i32, _vp]),
-    "lbm_dev_step_multi": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f32, _i32, _f32, _f32,
+    "lbm_dev_step_multi": (_i32, [_vp, _vp, _i32, _i64, _i64, _i32, _f64, _i32, _f64, _f64,
                                   _i32, _i32, ctypes.c_uint32, ctypes.c_uint32, _i32, _vp, _vp,
                                   _i32, _vp]),
     "lbm_dev_multi_variant_count": (_i32, []),
@@ -113,6 +114,9 @@ SIGNATURES = {
                               _vp, _i32, _i32, _vp]),
     "lbm_batch_amplitude": (_i32, [_vp, _i32, _f32, _vp, _i32, _vp, _i32, _vp, _vp, _vp, _i32,
                                    _vp, _vp]),
+    "lbm_batch_sweep_strips": (_i32, [_i32, _i32]),
+    "lbm_batch_step_multi": (_i32, [_vp, _vp, _i32, _f32, _vp, _vp, _i32, _i32, _i32, _i32,
+                                    ctypes.c_uint32, ctypes.c_uint32, _vp, _vp]),
     "lbm_counter_increment": (_i32, [_vp, _vp]),
     "lbm_counter_set": (_i32, [_vp, ctypes.c_uint32, _vp]),
     # HIP graphs
@@ -124,6 +128,10 @@ SIGNATURES = {
     "lbm_nn_gemm_workspace": (_i64, [_i32, _i32, _i32]),
     "lbm_nn_gemm": (_i32, [_vp, _i64, _i64, _vp, _i64, _i64, _vp, _vp, _i64, _vp, _i32, _i32,
                            _i32, _i32, _i32, _i32, _vp, _vp]),
+    "lbm_nn_conv3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "lbm_nn_conv3x3_wgrad": (_i32, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _i32, _vp, _vp]),
+    "lbm_nn_conv3x3_flip_weights": (_i32, [_vp, _i32, _i32, _vp, _vp]),
+    "lbm_nn_conv3x3_dgrad": (_i32, [_vp, _vp, _i32, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
     "lbm_nn_im2col3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _vp]),
     "lbm_nn_col2im3x3": (_i32, [_vp, _i32, _i32, _i32, _i32, _vp, _i32, _vp]),
     "lbm_nn_relu_backward": (_i32, [_vp, _vp, _vp, _i64, _vp]),
@@ -163,7 +171,11 @@ def load(path=LIB_PATH):
             "(the HIP kernels are the only implementation)")
     lib = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
-        fn = getattr(lib, name)
+        # a library built from older sources lacks the newer entry points:
+        # those raise when called (AttributeError), the others still load
+        fn = getattr(lib, name, None)
+        if fn is None:
+            continue
         fn.restype = res
         fn.argtypes = args
     _lib = lib

@@ -2,10 +2,13 @@
 
 ``BatchedLattices`` keeps hundreds of shear-wave (or cavity) instances of
 different sizes in one population pool (scaled deviations, row-interleaved
-layout per instance) and steps all of them with a few launches per step: one
-persistent launch per launch-shape group (``lbm_batch_step``).  The tile shape
-of each size class (nx x ny) is picked by the autotuner; classes that pick the
-same shape share a launch.  A chunk of steps plus the amplitude sample that
+layout per instance) and steps all of them together.  By default one launch
+advances every instance four steps (``lbm_batch_step_multi``: a work list of
+(instance, strip of columns, rows) items, one wave each, with the
+intermediate steps kept in registers), bitwise equal to four single steps;
+steps left over run one per launch through the single-step kernel
+(``lbm_batch_step``), one persistent launch per launch-shape group, whose
+tile shape per size class (nx x ny) the autotuner picks.  A chunk of steps plus the amplitude sample that
 ends it is captured once into a HIP graph and replayed, so a 2000-step sweep is
 a handful of graph launches.
 
@@ -25,10 +28,13 @@ from .precision import COMPUTE, DEFAULT_SCALE, FORMATS, ROUNDING, _RAW
 
 SIZES = (64, 128, 256, 512, 1024)
 
-INST_DTYPE = np.dtype([("offset", "<i8"), ("nx", "<i4"), ("ny", "<i4"), ("omega", "<f4"),
-                       ("u0", "<f4"), ("bc", "<i4"), ("dir", "<i4")])
+INST_DTYPE = np.dtype([("offset", "<i8"), ("nx", "<i4"), ("ny", "<i4"), ("omega", "<f8"),
+                       ("u0", "<f8"), ("bc", "<i4"), ("dir", "<i4")])
 TILE_DTYPE = np.dtype([("inst", "<i4"), ("first", "<i4")])
+ITEM_DTYPE = np.dtype([("inst", "<i4"), ("strip", "<i4"), ("row0", "<i4"), ("rows", "<i4")])
+SWEEP_ROWS = 64  # rows per work item of the K-step sweep
 AMP_TILE = 256  # nodes per amplitude tile (the reduction block size)
+SEED_STEP = 0x85EBCA77  # difference of the dither seeds of consecutive steps
 
 
 def shear_wave_sweep(n_instances=512, seed=0):
@@ -53,6 +59,20 @@ def _to_device_bytes(arr, device):
     return torch.from_numpy(np.ascontiguousarray(arr).view(np.uint8).copy()).to(device)
 
 
+def sweep_items(table, steps, rows=SWEEP_ROWS):
+    """Work list of the K-step batched sweep: for every instance, every strip
+    of columns (``lbm_batch_sweep_strips``) times every range of ``rows``
+    rows, each item worked by one wave."""
+    lib = nat.load()
+    out = []
+    for i in range(len(table)):
+        nx, ny = int(table["nx"][i]), int(table["ny"][i])
+        for strip in range(lib.lbm_batch_sweep_strips(steps, ny)):
+            for r0 in range(0, nx, rows):
+                out.append((i, strip, r0, min(rows, nx - r0)))
+    return np.array(out, dtype=np.int32).view(ITEM_DTYPE).reshape(-1)
+
+
 def _tiles(inst_ids, sizes, tile_nodes):
     rows = []
     for i in inst_ids:
@@ -67,7 +87,7 @@ class BatchedLattices:
     """Independent D2Q9 lattices stepped together on one GPU."""
 
     def __init__(self, instances, fmt="f32", scale=None, rounding="stochastic", seed=0,
-                 variant="auto", device=None, compute=None):
+                 variant="auto", device=None, compute=None, steps_per_launch=4):
         nat.require_gpu()
         # FP32 storage defaults to an FP64 collide with unbiased rounding: a
         # slowly decaying wave changes by only a few FP32 ulps per step
@@ -134,6 +154,16 @@ class BatchedLattices:
         for v, ids in sorted(groups.items()):
             t = _tiles(sorted(ids), self.sizes, lib.lbm_batch_tile_nodes(v))
             self.groups.append((v, _to_device_bytes(t, self.device), len(t)))
+        # K steps per launch: every instance in one work list, one launch
+        self.multi_k = int(steps_per_launch)
+        if self.multi_k not in (1, 2, 4):
+            raise ValueError("steps_per_launch must be 1, 2 or 4")
+        if self.multi_k > 1:
+            if any(int(p["nx"]) < 2 * self.multi_k for p in self.instances):
+                raise ValueError("instances need nx >= 2 * steps_per_launch")
+            items = sweep_items(table, self.multi_k)
+            self._items = _to_device_bytes(items, self.device)
+            self._nitems = len(items)
 
     # -- state -------------------------------------------------------------------
     def init_states(self):
@@ -188,19 +218,40 @@ class BatchedLattices:
                             "x%d" % len(ids), make_launch, range(n_var), reps=5)
 
     def _step_seed(self, s):
-        return (self.seed * 0x9E3779B1 + s * 0x85EBCA77 + 0x165667B1) & 0xFFFFFFFF
+        # consecutive steps differ by SEED_STEP, so a K-step launch takes the
+        # seed of its first step and SEED_STEP as the stride
+        return (self.seed * 0x9E3779B1 + s * SEED_STEP + 0x165667B1) & 0xFFFFFFFF
+
+    def _launch_multi(self, src, dst, seed):
+        nat.call("lbm_batch_step_multi", nat.ptr(src), nat.ptr(dst), self.fmt, self.scale,
+                 nat.ptr(self._insts), nat.ptr(self._items), self._nitems, self.multi_k,
+                 self.compute, self.rounding, seed, SEED_STEP, nat.ptr(self._epoch),
+                 nat.stream_ptr())
+
+    def _launches(self, n, first):
+        """(kind, step index) of the launches that advance n steps from step
+        ``first``: runs of multi_k steps through the sweep, the rest single."""
+        k = self.multi_k
+        runs = n // k if k > 1 else 0
+        out = [("multi", first + r * k) for r in range(runs)]
+        out += [("single", first + runs * k + s) for s in range(n - runs * k)]
+        return out
 
     def step(self, n=1):
         """Advance every instance n steps (eager launches)."""
-        for _ in range(n):
-            for v, tiles, nt in self.groups:
-                self._launch(self.a, self.b, v, tiles, nt, self._step_seed(self.time))
+        for kind, s in self._launches(n, self.time):
+            if kind == "multi":
+                self._launch_multi(self.a, self.b, self._step_seed(s))
+            else:
+                for v, tiles, nt in self.groups:
+                    self._launch(self.a, self.b, v, tiles, nt, self._step_seed(s))
             self.a, self.b = self.b, self.a
-            self.time += 1
+        self.time += n
         return self
 
-    def _amplitude(self, amp, nslots):
-        nat.call("lbm_batch_amplitude", nat.ptr(self.a), self.fmt, self.scale, nat.ptr(self._insts),
+    def _amplitude(self, amp, nslots, pool=None):
+        pool = self.a if pool is None else pool
+        nat.call("lbm_batch_amplitude", nat.ptr(pool), self.fmt, self.scale, nat.ptr(self._insts),
                  self.n, nat.ptr(self._amp_tiles), self._amp_ntiles, nat.ptr(self._inst_tiles),
                  nat.ptr(self._partial), nat.ptr(amp), nslots, nat.ptr(self._epoch),
                  nat.stream_ptr())
@@ -208,7 +259,9 @@ class BatchedLattices:
     def run(self, nsteps, sample_every=100, graph=True):
         """Run nsteps steps, sampling every instance's amplitude every
         ``sample_every`` steps (and at the start).  With ``graph`` the chunk
-        of ``sample_every`` steps + sample is captured once and replayed.
+        of ``sample_every`` steps + sample is captured once and replayed (as
+        two graphs used in turn when a chunk takes an odd number of launches,
+        so that every replay starts from the buffer holding the state).
 
         Returns (times, amplitudes) with amplitudes of shape (samples, n).
         """
@@ -221,41 +274,53 @@ class BatchedLattices:
         self._amplitude(amp, nslots)
         nat.call("lbm_counter_increment", nat.ptr(self._epoch), nat.stream_ptr())
         t0 = self.time
+        launches = self._launches(sample_every, 0)
+        odd = len(launches) % 2 == 1
 
-        def chunk():
-            # sample_every steps (an even number: the state ends in self.a),
-            # then the amplitude sample; the dither seed of a step is its index
-            # in the chunk combined on the device with the chunk counter
-            a, b = self.a, self.b
-            for s in range(sample_every):
-                for v, tiles, nt in self.groups:
-                    self._launch(a, b, v, tiles, nt, self._step_seed(s))
+        def chunk(a, b):
+            # sample_every steps, then the amplitude sample of the buffer that
+            # holds the state; the dither seed of a step is its index in the
+            # chunk combined on the device with the chunk counter
+            for kind, st in launches:
+                if kind == "multi":
+                    self._launch_multi(a, b, self._step_seed(st))
+                else:
+                    for v, tiles, nt in self.groups:
+                        self._launch(a, b, v, tiles, nt, self._step_seed(st))
                 a, b = b, a
-            self._amplitude(amp, nslots)
+            self._amplitude(amp, nslots, a)
             nat.call("lbm_counter_increment", nat.ptr(self._epoch), nat.stream_ptr())
 
+        starts = [(self.a, self.b), (self.b, self.a)]
+        order = [starts[c % 2] if odd else starts[0] for c in range(chunks)]
         if not graph:
-            for _ in range(chunks):
-                chunk()
+            for a, b in order:
+                chunk(a, b)
         else:
             side = torch.cuda.Stream(device=self.device)
             side.wait_stream(torch.cuda.current_stream(self.device))
             lib = nat.load()
-            g = nat.ctypes.c_void_p()
+            graphs = []
             with torch.cuda.stream(side):
-                nat.call("lbm_graph_begin", nat.stream_ptr())
                 try:
-                    chunk()
-                finally:
-                    status = lib.lbm_graph_end(nat.stream_ptr(), nat.ctypes.byref(g))
-                nat.check(status, "lbm_graph_end")
-                try:
-                    for _ in range(chunks):
-                        nat.call("lbm_graph_launch", g, nat.stream_ptr())
+                    for a, b in starts[:2 if odd and chunks > 1 else 1]:
+                        g = nat.ctypes.c_void_p()
+                        nat.call("lbm_graph_begin", nat.stream_ptr())
+                        try:
+                            chunk(a, b)
+                        finally:
+                            status = lib.lbm_graph_end(nat.stream_ptr(), nat.ctypes.byref(g))
+                        nat.check(status, "lbm_graph_end")
+                        graphs.append(g)
+                    for c in range(chunks):
+                        nat.call("lbm_graph_launch", graphs[c % len(graphs)], nat.stream_ptr())
                 finally:
                     side.synchronize()
-                    nat.check(lib.lbm_graph_destroy(g), "lbm_graph_destroy")
+                    for g in graphs:
+                        nat.check(lib.lbm_graph_destroy(g), "lbm_graph_destroy")
             torch.cuda.current_stream(self.device).wait_stream(side)
+        if odd and chunks % 2 == 1:
+            self.a, self.b = self.b, self.a
         self.time += nsteps
         times = t0 + sample_every * np.arange(nslots)
         return times, amp.cpu().numpy()

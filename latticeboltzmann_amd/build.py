@@ -37,8 +37,8 @@ COMMON = [
 # per-unit flags
 UNITS = {
     "d2q9_kernels.hip": ["-ffp-contract=off"],
-    "d2q9_lowprec.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
-    "d2q9_batch.hip": ["-ffp-contract=fast", "-fno-slp-vectorize"],
+    "d2q9_lowprec.hip": ["-ffp-contract=off", "-fno-slp-vectorize"],
+    "d2q9_batch.hip": ["-ffp-contract=off", "-fno-slp-vectorize"],
     "surrogate.hip": ["-ffp-contract=fast"],
     "ghost.hip": ["-ffp-contract=off"],
 }

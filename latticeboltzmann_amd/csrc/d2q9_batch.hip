@@ -89,6 +89,50 @@ __global__ __launch_bounds__(BS) void k_batch_step(const typename Codec<FMT>::ra
     }
 }
 
+// K steps of the instances of a work list per launch, by the register sweep of
+// d2q9_dev.hpp: each wave takes one item (instance, strip of columns, range
+// of rows), so one launch covers instances of every size.  Every step is
+// rounded with the batch's dither (the step's seed combined on the device
+// with *epoch; node = instance offset / 9 + k ny + l), so a launch is bitwise
+// K lbm_batch_step launches.  Instances need ny >= 64 and nx >= 2K.
+template <int FMT, typename CT, int K>
+__global__ __launch_bounds__(256) void k_batch_sweep(const typename Codec<FMT>::raw *__restrict__ src,
+                                                     typename Codec<FMT>::raw *__restrict__ dst,
+                                                     const lbm_batch_inst *__restrict__ insts,
+                                                     const lbm_batch_item *__restrict__ items,
+                                                     int nitems, int nblocks, CT scale, CT inv_scale,
+                                                     int stochastic, uint32_t seed,
+                                                     uint32_t seed_stride,
+                                                     const uint32_t *__restrict__ epoch) {
+    const int b = xcd_tile(nblocks);
+    const int w = __builtin_amdgcn_readfirstlane(b * 4 + int(threadIdx.x >> 6));
+    if (b >= nblocks || w >= nitems) return;  // whole waves; nothing waits for them
+    const lbm_batch_item it = items[w];
+    const lbm_batch_inst I = insts[it.inst];
+    const int nx = I.nx, ny = I.ny;
+    const Layout L{ny, 9 * int64_t(ny)};
+    const DevSweepLane ln = dev_sweep_lane<K>(it.strip, ny, int(threadIdx.x & 63));
+    const uint32_t mix = epoch ? epoch[0] * 0x9e3779b9U : 0u;
+    uint32_t seeds[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        const uint32_t sj = seed + uint32_t(j) * seed_stride;
+        seeds[j] = epoch ? hash32(sj ^ mix) : sj;
+    }
+    const auto *s = src + I.offset;
+    auto *d = dst + I.offset;
+    const uint64_t base = uint64_t(I.offset) / 9;
+    const int k1 = it.row0 + it.rows;
+    if (I.bc == LBM_BC_CAVITY)
+        dev_sweep_run<FMT, CT, LBM_BC_CAVITY, false, K, true>(
+            s, d, it.row0, k1, nx, ny, L, ln, CT(I.omega), CT(I.u0), scale, inv_scale, stochastic,
+            seeds, base, nullptr, nullptr);
+    else
+        dev_sweep_run<FMT, CT, LBM_BC_PERIODIC, false, K, true>(
+            s, d, it.row0, k1, nx, ny, L, ln, CT(I.omega), CT(I.u0), scale, inv_scale, stochastic,
+            seeds, base, nullptr, nullptr);
+}
+
 // Pass 1 of the amplitude: one kBlock-node tile per loop iteration, partial
 // sum of u_dir * sin(2 pi pos / n) in FP64.
 template <int FMT>
@@ -220,6 +264,39 @@ inline bool valid_fmt_b(int fmt) {
     return fmt == LBM_FMT_F32 || fmt == LBM_FMT_BF16 || fmt == LBM_FMT_F16;
 }
 
+template <int FMT, typename CT, int K>
+void launch_batch_sweep_v(const void *src, void *dst, float scale, const lbm_batch_inst *insts,
+                          const lbm_batch_item *items, int nitems, int stochastic, uint32_t seed,
+                          uint32_t stride, const uint32_t *epoch, hipStream_t s) {
+    using R = typename Codec<FMT>::raw;
+    const int nblocks = (nitems + 3) / 4;
+    const int grid = ((nblocks + 7) / 8) * 8;
+    hipLaunchKernelGGL((k_batch_sweep<FMT, CT, K>), dim3(unsigned(grid)), dim3(256), 0, s,
+                       static_cast<const R *>(src), static_cast<R *>(dst), insts, items, nitems,
+                       nblocks, CT(scale), CT(1) / CT(scale), stochastic, seed, stride, epoch);
+}
+
+template <typename CT, int K>
+void launch_batch_sweep_k(int fmt, const void *src, void *dst, float scale,
+                          const lbm_batch_inst *insts, const lbm_batch_item *items, int nitems,
+                          int stochastic, uint32_t seed, uint32_t stride, const uint32_t *epoch,
+                          hipStream_t s) {
+    switch (fmt) {
+        case LBM_FMT_F32:
+            launch_batch_sweep_v<LBM_FMT_F32, CT, K>(src, dst, scale, insts, items, nitems,
+                                                     stochastic, seed, stride, epoch, s);
+            break;
+        case LBM_FMT_BF16:
+            launch_batch_sweep_v<LBM_FMT_BF16, CT, K>(src, dst, scale, insts, items, nitems,
+                                                      stochastic, seed, stride, epoch, s);
+            break;
+        default:
+            launch_batch_sweep_v<LBM_FMT_F16, CT, K>(src, dst, scale, insts, items, nitems,
+                                                     stochastic, seed, stride, epoch, s);
+            break;
+    }
+}
+
 }  // namespace lbm
 
 using namespace lbm;
@@ -256,6 +333,43 @@ int lbm_batch_step(const void *src, void *dst, int fmt, float scale, const lbm_b
             case LBM_FMT_BF16: launch_batch_fmt<LBM_FMT_BF16, float>(a, variant, s); break;
             default: launch_batch_fmt<LBM_FMT_F16, float>(a, variant, s); break;
         }
+    }
+    return launch_status();
+}
+
+int lbm_batch_sweep_strips(int steps, int ny) {
+    if (steps == 2) return dev_sweep_strips<2>(ny);
+    if (steps == 4) return dev_sweep_strips<4>(ny);
+    return 0;
+}
+
+int lbm_batch_step_multi(const void *src, void *dst, int fmt, float scale,
+                         const lbm_batch_inst *insts, const lbm_batch_item *items, int nitems,
+                         int steps, int compute, int rounding, uint32_t seed, uint32_t seed_stride,
+                         const uint32_t *epoch, void *stream) {
+    if (!src || !dst || !insts || !items) return LBM_E_NULL;
+    if (src == dst) return LBM_E_ALIAS;
+    if (nitems <= 0 || nitems > (INT32_MAX - 31) / 4 * 4) return LBM_E_SIZE;
+    if (!valid_fmt_b(fmt) || !(scale > 0.0f)) return LBM_E_ARG;
+    if (steps != 2 && steps != 4) return LBM_E_ARG;
+    if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
+    if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
+    const int sr = rounding == LBM_ROUND_STOCHASTIC ? 1 : 0;
+    hipStream_t s = as_stream(stream);
+    if (compute == LBM_COMPUTE_F64) {
+        if (steps == 2)
+            launch_batch_sweep_k<double, 2>(fmt, src, dst, scale, insts, items, nitems, sr, seed,
+                                            seed_stride, epoch, s);
+        else
+            launch_batch_sweep_k<double, 4>(fmt, src, dst, scale, insts, items, nitems, sr, seed,
+                                            seed_stride, epoch, s);
+    } else {
+        if (steps == 2)
+            launch_batch_sweep_k<float, 2>(fmt, src, dst, scale, insts, items, nitems, sr, seed,
+                                           seed_stride, epoch, s);
+        else
+            launch_batch_sweep_k<float, 4>(fmt, src, dst, scale, insts, items, nitems, sr, seed,
+                                           seed_stride, epoch, s);
     }
     return launch_status();
 }

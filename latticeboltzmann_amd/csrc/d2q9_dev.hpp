@@ -1,14 +1,20 @@
 // Per-node arithmetic of the reduced-precision (scaled deviation) storage:
 // storage codecs, stochastic-rounding dither, cavity walls and BGK collide on
 // deviations, and the conservation-compensated rounding of a node's nine
-// values.  Shared by the single-lattice and the batched step kernels so both
-// compute the same bits; units including it are built with -ffp-contract=fast.
+// values.  Shared by the single-lattice and the batched step kernels so all of
+// them compute the same bits.  Units including it are built with
+// -ffp-contract=off and the fused multiply-adds of the collide are written
+// out (fma_ct): which products the compiler would fuse depends on the code
+// around the collide (values in flight from memory, held in registers since an
+// earlier step, or exchanged between lanes), so kernels that must agree bit for
+// bit would otherwise round differently.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/lbm_d2q9.h"
+#include "d2q9_grid.hpp"
 
 namespace lbm {
 
@@ -209,32 +215,63 @@ __device__ __forceinline__ void cavity_walls_ct(CT g[9], const CT *c, int64_t cs
     }
 }
 
-// BGK collide on deviations in the compute type CT (float or double).
+// a b + c with one rounding
+__device__ __forceinline__ float fma_ct(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_ct(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// 1 / x: the hardware reciprocal in FP32 (within 1 ulp), the IEEE quotient in FP64.
+__device__ __forceinline__ float recip_ct(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ double recip_ct(double x) { return 1.0 / x; }
+
+// x with the lowest significand bit cleared.
+__device__ __forceinline__ float clear_lsb(float x) {
+    return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) & ~1u);
+}
+__device__ __forceinline__ double clear_lsb(double x) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, x) & ~uint64_t(1));
+}
+
+// BGK collide on deviations in the compute type CT (float or double):
+// d_i <- (1 - omega) d_i + omega deq_i with
+//   omega deq_i = omega w_i (a + 4.5 rho cu_i^2) + 3 omega w_i (c_i . j),
+//   a = drho - 1.5 rho u^2, cu_i = c_i . u.
+// Opposite channels share the even part and take the odd part with opposite
+// signs (about 60 operations per node).  The odd part uses the momentum j
+// itself, with coefficients b1 (axes) and b5 (diagonals) built so that
+// b1 + 2 b5 = omega / 2 holds exactly: with 3 w_i rounded independently
+// (fl(1/3), fl(1/12)) each FP32 collide would scale the momentum by the same
+// 1 + O(1e-8), a drift that biases the decay of a slow wave by several percent.
+// u = j / rho is evaluated as j - j (drho / rho), so the reciprocal's error
+// only touches the small correction.
 template <typename CT>
 __device__ __forceinline__ void collide_dev(CT d[9], CT omega, CT &ux, CT &uy) {
-    const CT drho = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + (d[7] + d[8])));
+    const CT d78 = d[7] + d[8];
+    const CT drho = ((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d78));
     const CT rho = CT(1) + drho;
     const CT jx = ((d[1] - d[3]) + (d[5] - d[6])) + (d[8] - d[7]);
-    const CT jy = ((d[2] - d[4]) + (d[5] + d[6])) - (d[7] + d[8]);
-    const CT inv = CT(1) / rho;
-    ux = jx * inv;
-    uy = jy * inv;
-    const CT uu15 = CT(1.5) * (ux * ux + uy * uy);
-    CT cu[9];
-    cu[0] = CT(0);
-    cu[1] = ux;
-    cu[2] = uy;
-    cu[3] = -ux;
-    cu[4] = -uy;
-    cu[5] = ux + uy;
-    cu[6] = uy - ux;
-    cu[7] = -(ux + uy);
-    cu[8] = ux - uy;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        const CT deq = wt<CT>(i) * (drho + rho * ((CT(3) * cu[i] + CT(4.5) * (cu[i] * cu[i])) - uu15));
-        d[i] = d[i] + omega * (deq - d[i]);
-    }
+    const CT jy = ((d[2] - d[4]) + (d[5] + d[6])) - d78;
+    const CT q = drho * recip_ct(rho);
+    ux = fma_ct(-jx, q, jx);
+    uy = fma_ct(-jy, q, jy);
+    const CT a = fma_ct(CT(-1.5) * rho, fma_ct(ux, ux, uy * uy), drho);
+    const CT keep = CT(1) - omega;
+    const CT r45 = CT(4.5) * rho;
+    const CT ow0 = omega * wt<CT>(0), ow1 = omega * wt<CT>(1), ow5 = omega * wt<CT>(5);
+    const CT e1 = ow1 * r45, a1 = ow1 * a;  // even part, axis channels
+    const CT e5 = ow5 * r45, a5 = ow5 * a;  // even part, diagonals
+    // b5 ~ omega / 12 on a significand of one bit less, so that omega / 2 - 2 b5 is exact
+    const CT b5 = clear_lsb(omega * (CT(1) / CT(12)));
+    const CT b1 = fma_ct(CT(-2), b5, CT(0.5) * omega);
+    d[0] = fma_ct(keep, d[0], ow0 * a);
+    auto pair = [&](int i, int j, CT cu, CT cj, CT e, CT a_, CT b) {
+        const CT even = fma_ct(e, cu * cu, a_);
+        d[i] = fma_ct(b, cj, fma_ct(keep, d[i], even));
+        d[j] = fma_ct(-b, cj, fma_ct(keep, d[j], even));
+    };
+    pair(1, 3, ux, jx, e1, a1, b1);
+    pair(2, 4, uy, jy, e1, a1, b1);
+    pair(5, 7, ux + uy, jx + jy, e5, a5, b5);
+    pair(8, 6, ux - uy, jx - jy, e5, a5, b5);
 }
 
 // Round one value of the compute type to the storage format: nearest, or
@@ -309,6 +346,209 @@ __device__ __forceinline__ void store_values(const CT g[9], CT scale, int stocha
         if (stochastic) round_conserving<FMT, true, CT>(x, out, dither);
         else round_conserving<FMT, false, CT>(x, out, dither);
     }
+}
+
+// ---------------------------------------------------------------------------
+// K steps per launch on stored deviations by a sweep along x with the
+// intermediate steps in registers (the single-lattice k_dev_sweep and the
+// batched k_batch_sweep): a wave owns 64 columns and a segment of rows of one
+// lattice; per iteration it computes step 1 of a new row from src, then step j
+// of the row behind step j - 1's newest, j = 2..K, and writes step K's row;
+// column neighbours come from the adjacent lanes through ds_bpermute.
+// EACH: every step is rounded to the storage format with its own step's dither
+// (seeds[j - 1], node node_base + k ny + l) and the next step reads the decoded
+// stored value, so the launch is bitwise K single-step launches; otherwise
+// (store every K) the intermediate steps stay unrounded in the compute type and
+// only step K is rounded, with seeds[K - 1].
+__device__ __forceinline__ float dev_lane_fetch(float v, int addr) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double dev_lane_fetch(double v, int addr) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u))));
+    const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u >> 32))));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+// cavity_walls_ct with the bottom and lid rules behind the wave-uniform test
+// yedge (whether the wave's columns, halo included, reach l = 0 or ny - 1).
+template <typename CT>
+__device__ __forceinline__ void dev_sweep_walls(CT g[9], const CT c[9], int k, int l, int nx,
+                                                int ny, CT u0, bool yedge) {
+    if (yedge) {
+        if (l == 0) {
+            g[2] = c[4];
+            g[5] = c[7];
+            g[6] = c[8];
+        }
+        if (l == ny - 1) {
+            const CT c2 = c[2], c5 = c[5], c6 = c[6];
+            const CT rl = CT(1) + (((c6 + c2) + (c5 + g[6])) + ((g[2] + g[5]) + (g[3] + (g[0] + g[1]))));
+            const CT t = (CT(6) / CT(36)) * rl * u0;
+            g[4] = c2;
+            g[8] = c6 + t;
+            g[7] = c5 - t;
+        }
+    }
+    if (k == 0) {
+        g[1] = c[3];
+        g[5] = c[7];
+        g[8] = c[6];
+    }
+    if (k == nx - 1) {
+        g[3] = c[1];
+        g[6] = c[8];
+        g[7] = c[5];
+    }
+}
+
+struct DevSweepLane {
+    int l, from_lo, from_hi;
+    bool writer, yedge;
+};
+
+template <int FMT, typename CT, int BC, bool MOM, int K, bool EACH, int PH>
+__device__ __forceinline__ void dev_sweep_iteration(
+    const typename Codec<FMT>::raw *__restrict__ src, typename Codec<FMT>::raw *__restrict__ dst,
+    int i, int ks, int nx, int ny, Layout L, const DevSweepLane &ln, CT omega, CT u0, CT scale,
+    CT inv_scale, int stochastic, const uint32_t (&seeds)[K], uint64_t node_base,
+    CT (&A)[K - 1][9], CT (&B)[K - 1][9], float *__restrict__ ux_o, float *__restrict__ uy_o) {
+    using C = Codec<FMT>;
+    using R = typename C::raw;
+    CT g[9];
+    {  // step 1 of row i + K - 1, from src
+        const int k = wrap1(i + K - 1, nx);
+        const PullRows<R> rows(src, L, k, nx);
+        R raw[9];
+        pull9<false>(rows, ln.l, ny, raw);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = CT(C::dec(raw[q])) * inv_scale;
+        if constexpr (BC == LBM_BC_CAVITY) {
+            if (ln.yedge || k == 0 || k == nx - 1)
+                cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + ln.l, L.cs, k, ln.l, nx, ny,
+                                          u0, inv_scale);
+        }
+        CT ux, uy;
+        collide_dev<CT>(g, omega, ux, uy);
+        if constexpr (EACH) {
+            Dither dither(seeds[0], node_base + uint64_t(k) * uint64_t(ny) + uint64_t(ln.l));
+            R q[9];
+            store_values<FMT, CT>(g, scale, stochastic, dither, q);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) g[c] = CT(C::dec(q[c])) * inv_scale;
+        }
+    }
+#pragma unroll
+    for (int j = 2; j <= K; ++j) {
+        CT(&older)[9] = PH ? B[j - 2] : A[j - 2];
+        CT(&newer)[9] = PH ? A[j - 2] : B[j - 2];
+        const bool active = i >= ks - 2 * (K - j);  // wave-uniform
+        CT h[9];
+        int k = 0;
+        if (active) {
+            k = wrap1(i + K - j, nx);
+            h[0] = newer[0];
+            h[1] = older[1];
+            h[2] = dev_lane_fetch(newer[2], ln.from_lo);
+            h[3] = g[3];
+            h[4] = dev_lane_fetch(newer[4], ln.from_hi);
+            h[5] = dev_lane_fetch(older[5], ln.from_lo);
+            h[6] = dev_lane_fetch(g[6], ln.from_lo);
+            h[7] = dev_lane_fetch(g[7], ln.from_hi);
+            h[8] = dev_lane_fetch(older[8], ln.from_hi);
+            if constexpr (BC == LBM_BC_CAVITY) dev_sweep_walls<CT>(h, newer, k, ln.l, nx, ny, u0, ln.yedge);
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) older[q] = g[q];
+        if (!active) return;
+        CT ux, uy;
+        collide_dev<CT>(h, omega, ux, uy);
+        const uint32_t sj = seeds[j - 1];
+        if (j == K) {
+            if (ln.writer) {
+                Dither dither(sj, node_base + uint64_t(k) * uint64_t(ny) + uint64_t(ln.l));
+                R q[9];
+                store_values<FMT, CT>(h, scale, stochastic, dither, q);
+                R *d = dst + int64_t(k) * L.rs + ln.l;
+#pragma unroll
+                for (int c = 0; c < 9; ++c) d[c * L.cs] = q[c];
+                if constexpr (MOM) {
+                    const int64_t m = int64_t(k) * ny + ln.l;
+                    ux_o[m] = float(ux);
+                    uy_o[m] = float(uy);
+                }
+            }
+        } else if constexpr (EACH) {
+            Dither dither(sj, node_base + uint64_t(k) * uint64_t(ny) + uint64_t(ln.l));
+            R q[9];
+            store_values<FMT, CT>(h, scale, stochastic, dither, q);
+#pragma unroll
+            for (int c = 0; c < 9; ++c) g[c] = CT(C::dec(q[c])) * inv_scale;
+        } else {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) g[c] = h[c];
+        }
+    }
+}
+
+// The rows [ks, ke) of one strip (described by ln) of one lattice.
+template <int FMT, typename CT, int BC, bool MOM, int K, bool EACH>
+__device__ __forceinline__ void dev_sweep_run(const typename Codec<FMT>::raw *__restrict__ src,
+                                              typename Codec<FMT>::raw *__restrict__ dst, int ks,
+                                              int ke, int nx, int ny, Layout L,
+                                              const DevSweepLane &ln, CT omega, CT u0, CT scale,
+                                              CT inv_scale, int stochastic,
+                                              const uint32_t (&seeds)[K], uint64_t node_base,
+                                              float *__restrict__ ux_o, float *__restrict__ uy_o) {
+    CT A[K - 1][9], B[K - 1][9];
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) A[j][q] = B[j][q] = CT(0);
+    int i = ks - 2 * (K - 1);
+    for (; i + 1 < ke; i += 2) {
+        dev_sweep_iteration<FMT, CT, BC, MOM, K, EACH, 0>(src, dst, i, ks, nx, ny, L, ln, omega, u0,
+                                                          scale, inv_scale, stochastic, seeds,
+                                                          node_base, A, B, ux_o, uy_o);
+        dev_sweep_iteration<FMT, CT, BC, MOM, K, EACH, 1>(src, dst, i + 1, ks, nx, ny, L, ln, omega,
+                                                          u0, scale, inv_scale, stochastic, seeds,
+                                                          node_base, A, B, ux_o, uy_o);
+    }
+    if (i < ke)
+        dev_sweep_iteration<FMT, CT, BC, MOM, K, EACH, 0>(src, dst, i, ks, nx, ny, L, ln, omega, u0,
+                                                          scale, inv_scale, stochastic, seeds,
+                                                          node_base, A, B, ux_o, uy_o);
+}
+
+// Lane set-up of a wave on strip `strip` of an ny-column lattice (VW =
+// 64 - 2(K-1) written columns per strip).  A lattice of exactly 64 columns is
+// one strip with no halo lanes: its column neighbours wrap within the wave
+// (lane 63's upper neighbour is lane 0, column 0), which is the periodic pull
+// the walls then override, so all 64 lanes are written.
+template <int K>
+__device__ __forceinline__ DevSweepLane dev_sweep_lane(int strip, int ny, int lane) {
+    constexpr int VW = 64 - 2 * (K - 1);
+    DevSweepLane ln;
+    ln.from_lo = ((lane + 63) & 63) << 2;
+    ln.from_hi = ((lane + 1) & 63) << 2;
+    if (ny == 64) {
+        ln.l = lane;
+        ln.writer = true;
+        ln.yedge = true;
+        return ln;
+    }
+    const int c = strip * VW - (K - 1) + lane;  // in [-(K-1), ny + 63): one wrap suffices
+    ln.l = wrap1(c, ny);
+    ln.writer = lane >= K - 1 && lane < 64 - (K - 1) && c < ny;
+    const int c0 = strip * VW - (K - 1);  // the wave's columns before wrapping: c0 .. c0 + 63
+    ln.yedge = c0 <= 0 || c0 + 63 >= ny - 1;
+    return ln;
+}
+
+// Strips of an ny-column lattice.
+template <int K>
+__host__ __device__ constexpr int dev_sweep_strips(int ny) {
+    return ny == 64 ? 1 : (ny + (64 - 2 * (K - 1)) - 1) / (64 - 2 * (K - 1));
 }
 
 }  // namespace lbm

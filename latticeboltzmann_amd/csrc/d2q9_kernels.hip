@@ -255,6 +255,240 @@ __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__re
     phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
 }
 
+// ---------------------------------------------------------------------------
+// K fused steps per launch by a sweep along x, the intermediate steps kept in
+// registers (k_sweep).  A wave owns 64 consecutive columns (y) and a segment
+// of rows (x) and walks the rows in order.  Iteration i computes step 1 of
+// row i + K - 1 from src (pull, walls, collide: the single-step node code),
+// then step j of row i + K - j from step j - 1's rows i + K - j - 1 .. + 1 for
+// j = 2..K, and writes step K's row i to dst.  Each lane keeps, for every
+// intermediate step, its column's values of the two newest rows that step has
+// made, in two register buffers used in turn (the newest row goes over the
+// oldest once the next step has pulled from it), so no values are copied
+// between iterations.  Column neighbours (l - 1, l + 1) come from the
+// adjacent lanes through ds_bpermute (the LDS crossbar: no LDS memory is
+// allocated).  Step j is valid on lanes j - 1 .. 64 - j, so a wave writes the
+// middle 64 - 2 (K - 1) columns; its first 2 (K - 1) iterations fill the
+// pipeline.  Rows and columns are wrapped like the tile kernel's halo, so the
+// walls, the lid and its x-wrapped corner densities come out bitwise as K
+// single-step launches.  Waves never wait for each other: no barrier, no LDS.
+
+// Value of `v` held by the lane whose byte address (4 * lane) is `addr`.
+__device__ __forceinline__ double lane_fetch(double v, int addr) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u))));
+    const unsigned hi = unsigned(__builtin_amdgcn_ds_bpermute(addr, int(unsigned(u >> 32))));
+    return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float lane_fetch(float v, int addr) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(addr, __builtin_bit_cast(int, v)));
+}
+
+template <typename T>
+struct SweepLane {
+    int l;                 // the lane's column (wrapped)
+    int from_lo, from_hi;  // ds_bpermute addresses of lanes - 1 and + 1
+    bool writer;           // the lane's column is one the wave writes
+    bool yedge;            // wave-uniform: the wave's columns include l = 0 or l = ny - 1
+};
+
+// cavity_walls for the sweep: the same rules in the same order, with the
+// bottom and lid rules behind the wave-uniform test `yedge` (whether the
+// wave's columns, halo lanes included, reach l = 0 or l = ny - 1), so
+// interior waves branch past them instead of evaluating them for every lane.
+template <typename T>
+__device__ __forceinline__ void sweep_walls(T g[9], const T *c, int64_t cs, int k, int l, int nx,
+                                            int ny, T u0, bool yedge) {
+    if (yedge) {
+        if (l == 0) {
+            g[2] = c[4 * cs];
+            g[5] = c[7 * cs];
+            g[6] = c[8 * cs];
+        }
+        if (l == ny - 1) {
+            const T c2 = c[2 * cs], c5 = c[5 * cs], c6 = c[6 * cs];
+            T rl = c6 + c2;
+            rl = rl + c5;
+            rl = rl + g[6];
+            rl = rl + g[2];
+            rl = rl + g[5];
+            rl = rl + g[3];
+            rl = rl + g[0];
+            rl = rl + g[1];
+            const T t = (Weights<T>::lid * rl) * u0;
+            g[4] = c2;
+            g[8] = c6 + t;
+            g[7] = c5 - t;
+        }
+    }
+    if (k == 0) {
+        g[1] = c[3 * cs];
+        g[5] = c[7 * cs];
+        g[8] = c[6 * cs];
+    }
+    if (k == nx - 1) {
+        g[3] = c[1 * cs];
+        g[6] = c[8 * cs];
+        g[7] = c[5 * cs];
+    }
+}
+
+// Pulls of the step-1 row i + K - 1 from src (the loads of one iteration).
+template <typename T, int K>
+__device__ __forceinline__ void sweep_pull(const T *__restrict__ src, int i, int nx, int ny,
+                                           Layout L, const SweepLane<T> &ln, T g[9]) {
+    const PullRows<T> rows(src, L, wrap1(i + K - 1, nx), nx);
+    pull9<false>(rows, ln.l, ny, g);
+}
+
+// Register state of the sweep.  Step j (1 .. K-1) keeps its two newest rows.
+// PP (ping-pong): two full buffers per step used in turn, S[PH][j] the older
+// row and S[PH ^ 1][j] the newer; the row made next goes over the older one,
+// so nothing is copied.  Otherwise the newer row is S[0][j] and the older is
+// kept only in the three channels a pull takes from it (E, NE, SE: O[j]), and
+// the rows move down by register copies: a third less state for 12 moves
+// per step and iteration.  P holds step-1 pulls: with PF (prefetch) the
+// next iteration's are issued into P[PH ^ 1] before this iteration's
+// arithmetic, so their HBM latency overlaps it.
+template <typename T, int K>
+struct SweepState {
+    T S[2][K - 1][9];
+    T O[K - 1][3];
+    T P[2][9];
+};
+
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF, int PH>
+__device__ __forceinline__ void sweep_iteration(const T *__restrict__ src, T *__restrict__ dst,
+                                                int i, int ks, int ke, int nx, int ny, Layout L,
+                                                const SweepLane<T> &ln, T omega, T u0,
+                                                SweepState<T, K> &st, T *__restrict__ rho_o,
+                                                T *__restrict__ ux_o, T *__restrict__ uy_o) {
+    constexpr int cur = PF ? PH : 0;
+    if constexpr (PF) {
+        if (i + 1 < ke) sweep_pull<T, K>(src, i + 1, nx, ny, L, ln, st.P[PH ^ 1]);
+    } else {
+        sweep_pull<T, K>(src, i, nx, ny, L, ln, st.P[0]);
+    }
+    // step 1 of row i + K - 1, from its pulls
+    T g[9];
+    {
+        const int k = wrap1(i + K - 1, nx);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) g[q] = st.P[cur][q];
+        if constexpr (BC == LBM_BC_CAVITY)
+            sweep_walls(g, src + int64_t(k) * L.rs + ln.l, L.cs, k, ln.l, nx, ny, u0, ln.yedge);
+        T rho, ux, uy;
+        collide_opt1(g, omega, rho, ux, uy);
+    }
+#pragma unroll
+    for (int j = 2; j <= K; ++j) {
+        // step j - 1's rows: older (r - 1), newer (r), and g (r + 1, made just now)
+        T(&newer)[9] = PP ? st.S[PH ^ 1][j - 2] : st.S[0][j - 2];
+        const T o1 = PP ? st.S[PH][j - 2][1] : st.O[j - 2][0];
+        const T o5 = PP ? st.S[PH][j - 2][5] : st.O[j - 2][1];
+        const T o8 = PP ? st.S[PH][j - 2][8] : st.O[j - 2][2];
+        const bool active = i >= ks - 2 * (K - j);  // wave-uniform: the pipeline has filled
+        T h[9];
+        int k = 0;
+        if (active) {
+            k = wrap1(i + K - j, nx);
+            h[0] = newer[0];
+            h[1] = o1;                                 // E  from (k-1, l)
+            h[2] = lane_fetch(newer[2], ln.from_lo);   // N  from (k,   l-1)
+            h[3] = g[3];                               // W  from (k+1, l)
+            h[4] = lane_fetch(newer[4], ln.from_hi);   // S  from (k,   l+1)
+            h[5] = lane_fetch(o5, ln.from_lo);         // NE from (k-1, l-1)
+            h[6] = lane_fetch(g[6], ln.from_lo);       // NW from (k+1, l-1)
+            h[7] = lane_fetch(g[7], ln.from_hi);       // SW from (k+1, l+1)
+            h[8] = lane_fetch(o8, ln.from_hi);         // SE from (k-1, l+1)
+            if constexpr (BC == LBM_BC_CAVITY) sweep_walls(h, newer, 1, k, ln.l, nx, ny, u0, ln.yedge);
+        }
+        // the rows of step j - 1 move down: the new row replaces the older
+        if constexpr (PP) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) st.S[PH][j - 2][q] = g[q];
+        } else {
+            st.O[j - 2][0] = newer[1];
+            st.O[j - 2][1] = newer[5];
+            st.O[j - 2][2] = newer[8];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) newer[q] = g[q];
+        }
+        if (!active) return;  // later steps are not active either
+        T rho, ux, uy;
+        collide_opt1(h, omega, rho, ux, uy);
+        if (j == K) {
+            if (ln.writer && i >= ks) {
+                T *d = dst + int64_t(k) * L.rs + ln.l;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) d[q * L.cs] = h[q];
+                if constexpr (MOM) {
+                    const int64_t m = int64_t(k) * ny + ln.l;
+                    rho_o[m] = rho;
+                    ux_o[m] = ux;
+                    uy_o[m] = uy;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) g[q] = h[q];
+        }
+    }
+}
+
+// Grid: blocks of 4 waves, numbered XCD-contiguously (xcd_tile over blocks);
+// wave w takes segment w / nstrips (rows [seg * seg_rows, + seg_rows)) and
+// strip w % nstrips (columns [strip * (64 - 2(K-1)), + 64 - 2(K-1))).
+// Requires ny >= 64 and nx >= K.
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF>
+__global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ src, T *__restrict__ dst,
+                                               int nx, int ny, Layout L, int nstrips, int seg_rows,
+                                               int nwaves, int nblocks, T omega, T u0,
+                                               T *__restrict__ rho_o, T *__restrict__ ux_o,
+                                               T *__restrict__ uy_o) {
+    static_assert(K >= 2 && K <= 6, "2 to 6 steps per launch");
+    constexpr int VW = 64 - 2 * (K - 1);
+    const int b = xcd_tile(nblocks);
+    // the wave index is the same in all lanes: in a scalar register, the row
+    // loop, the wall tests of a row and the pipeline tests stay scalar
+    const int wave = __builtin_amdgcn_readfirstlane(b * 4 + int(threadIdx.x >> 6));
+    if (b >= nblocks || wave >= nwaves) return;  // whole waves; nothing waits for them
+    const int lane = int(threadIdx.x & 63);
+    const int seg = wave / nstrips, strip = wave - seg * nstrips;
+    const int ks = seg * seg_rows;
+    const int ke = min(ks + seg_rows, nx);
+    const int c = strip * VW - (K - 1) + lane;  // in [-(K-1), ny + 63): one wrap suffices
+    SweepLane<T> ln;
+    ln.l = wrap1(c, ny);
+    ln.from_lo = ((lane + 63) & 63) << 2;
+    ln.from_hi = ((lane + 1) & 63) << 2;
+    ln.writer = lane >= K - 1 && lane < 64 - (K - 1) && c < ny;
+    // the wave's columns c0 .. c0 + 63 (before wrapping) include l = 0 or
+    // l = ny - 1 when they reach 0 or ny - 1: the first strip, and every strip
+    // whose halo lanes run past the last column
+    const int c0 = strip * VW - (K - 1);
+    ln.yedge = c0 <= 0 || c0 + 63 >= ny - 1;
+    SweepState<T, K> st;
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) {
+#pragma unroll
+        for (int q = 0; q < 9; ++q) st.S[0][j][q] = st.S[1][j][q] = T(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) st.O[j][q] = T(0);
+    }
+    int i = ks - 2 * (K - 1);
+    if constexpr (PF) sweep_pull<T, K>(src, i, nx, ny, L, ln, st.P[0]);
+    for (; i + 1 < ke; i += 2) {
+        sweep_iteration<T, BC, MOM, K, PP, PF, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
+                                                  st, rho_o, ux_o, uy_o);
+        sweep_iteration<T, BC, MOM, K, PP, PF, 1>(src, dst, i + 1, ks, ke, nx, ny, L, ln, omega,
+                                                  u0, st, rho_o, ux_o, uy_o);
+    }
+    if (i < ke)
+        sweep_iteration<T, BC, MOM, K, PP, PF, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
+                                                  st, rho_o, ux_o, uy_o);
+}
+
 // Streaming with cavity walls, no collide (stream_and_bounce_back).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_stream_bounce(const T *__restrict__ src,
@@ -530,12 +764,39 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
     X(18, 4, 20, 32, 512)     \
     X(19, 4, 24, 28, 512)
 #define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
-constexpr int kNumMultiShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
+constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
+
+// Shapes of the register sweep (k_sweep), variants kNumTileShapes + id:
+// (steps per launch K, rows per wave segment, ping-pong buffers, prefetch).
+#define LBM_SWEEP_SHAPES(X)    \
+    X(0, 4, 16, true, true)    \
+    X(1, 4, 32, true, true)    \
+    X(2, 4, 32, false, true)   \
+    X(3, 4, 64, false, false)  \
+    X(4, 5, 32, false, false)  \
+    X(5, 5, 64, true, false)   \
+    X(6, 5, 64, false, false)  \
+    X(7, 5, 64, false, true)   \
+    X(8, 6, 64, false, false)  \
+    X(9, 3, 32, false, true)
+#define LBM_COUNT_SWEEP(id, k, seg, pp, pf) +1
+constexpr int kNumSweepShapes = 0 LBM_SWEEP_SHAPES(LBM_COUNT_SWEEP);
+constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
 // Four steps on 20 x 32 tiles with 512 threads: the fastest FP64 shape measured
 // (100-102 GLUPS at 2048^2, 105-106 at 8192^2; 71.1 KB of LDS, so two blocks
 // per CU).  Its four rings (26x38, 24x36, 22x34, 20x32 nodes) take two passes
 // of the block each, with few idle lanes in the second.
 constexpr int kDefaultMultiShape = 18;
+// From 4096^2 nodes on, the five-step sweep with 64-row segments and ping-pong
+// row buffers: 131-141 GLUPS at 8192^2 against 115-121 for the tile.  On
+// smaller grids its waves (one per 58 columns x 64 rows) are too few to hide
+// memory latency (83 GLUPS at 2048^2).
+constexpr int kLargeGridMultiShape = kNumTileShapes + 5;
+constexpr int64_t kLargeGridNodes = int64_t(4096) * 4096;
+
+inline int default_multi_shape(int64_t nx, int64_t ny) {
+    return nx * ny >= kLargeGridNodes ? kLargeGridMultiShape : kDefaultMultiShape;
+}
 
 template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
 int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
@@ -551,16 +812,36 @@ int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T ome
     return LBM_OK;
 }
 
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF>
+int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int seg_rows, T omega,
+                   T u0, T *rho, T *ux, T *uy, hipStream_t s) {
+    constexpr int VW = 64 - 2 * (K - 1);
+    if (nx < 2 * K || ny < 64) return LBM_E_SIZE;
+    const int64_t nstrips = (ny + VW - 1) / VW, nsegs = (nx + seg_rows - 1) / seg_rows;
+    const int64_t nwaves = nstrips * nsegs;
+    const int64_t nblocks = (nwaves + 3) / 4;
+    const int64_t grid = ((nblocks + 7) / 8) * 8;
+    if (grid > 0x7fffffffLL) return LBM_E_SIZE;
+    hipLaunchKernelGGL((k_sweep<T, BC, MOM, K, PP, PF>), dim3(unsigned(grid)), dim3(256), 0, s, src, dst,
+                       int(nx), int(ny), L, int(nstrips), seg_rows, int(nwaves), int(nblocks),
+                       omega, u0, rho, ux, uy);
+    return LBM_OK;
+}
+
 template <typename T, int BC, bool MOM>
 int launch_multi_bc(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
                     T *ux, T *uy, int shape, hipStream_t s) {
 #define LBM_LAUNCH_MULTI(id, k, tr, tc, bs) \
     case id: return launch_multi_v<T, BC, MOM, k, tr, tc, bs>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s);
+#define LBM_LAUNCH_SWEEP(id, k, seg, pp, pf) \
+    case kNumTileShapes + id: return launch_sweep_v<T, BC, MOM, k, pp, pf>(src, dst, nx, ny, L, seg, omega, u0, rho, ux, uy, s);
     switch (shape) {
         LBM_MULTI_SHAPES(LBM_LAUNCH_MULTI)
+        LBM_SWEEP_SHAPES(LBM_LAUNCH_SWEEP)
         default: return LBM_E_ARG;
     }
 #undef LBM_LAUNCH_MULTI
+#undef LBM_LAUNCH_SWEEP
 }
 
 template <typename T>
@@ -575,7 +856,7 @@ int multi_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega
     if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
     const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int shape = variant < 0 ? kDefaultMultiShape : variant;
+    const int shape = variant < 0 ? default_multi_shape(nx, ny) : variant;
     hipStream_t s = as_stream(stream);
     const Layout L = make_layout(layout, nx, ny);
     int rc;
@@ -592,11 +873,15 @@ inline int multi_shape_info(int variant, int *steps, int64_t *min_nx, int64_t *m
     const int shape = variant < 0 ? kDefaultMultiShape : variant;
 #define LBM_INFO_MULTI(id, k, tr, tc, bs) \
     case id: *steps = k; *min_nx = tr + 2 * k; *min_ny = tc + 2 * k; return LBM_OK;
+#define LBM_INFO_SWEEP(id, k, seg, pp, pf) \
+    case kNumTileShapes + id: *steps = k; *min_nx = 2 * k; *min_ny = 64; return LBM_OK;
     switch (shape) {
         LBM_MULTI_SHAPES(LBM_INFO_MULTI)
+        LBM_SWEEP_SHAPES(LBM_INFO_SWEEP)
         default: return LBM_E_ARG;
     }
 #undef LBM_INFO_MULTI
+#undef LBM_INFO_SWEEP
 }
 
 template <typename T>
@@ -738,6 +1023,11 @@ int lbm_step_variant_count(void) { return kNumStepShapes; }
 int lbm_step_multi_variant_count(void) { return kNumMultiShapes; }
 
 int lbm_step_multi_default_variant(void) { return kDefaultMultiShape; }
+
+int lbm_step_multi_default_for(int64_t nx, int64_t ny) {
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    return default_multi_shape(nx, ny);
+}
 
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
     if (!steps || !min_nx || !min_ny) return LBM_E_NULL;

@@ -385,6 +385,27 @@ __global__ __launch_bounds__(BS) void k_dev_stepk(const typename Codec<FMT>::raw
                                                       K == 2 ? s1 : (K == 3 ? s2 : s3), ux_o, uy_o);
 }
 
+// Grid and numbering as k_sweep.  Requires ny >= 64 and nx >= K.
+template <int FMT, typename CT, int BC, bool MOM, int K, bool EACH>
+__global__ __launch_bounds__(256) void k_dev_sweep(
+    const typename Codec<FMT>::raw *__restrict__ src, typename Codec<FMT>::raw *__restrict__ dst,
+    int nx, int ny, Layout L, int nstrips, int seg_rows, int nwaves, int nblocks, CT omega, CT u0,
+    CT scale, CT inv_scale, int stochastic, uint32_t seed, uint32_t seed_stride,
+    float *__restrict__ ux_o, float *__restrict__ uy_o) {
+    const int b = xcd_tile(nblocks);
+    const int wave = __builtin_amdgcn_readfirstlane(b * 4 + int(threadIdx.x >> 6));
+    if (b >= nblocks || wave >= nwaves) return;  // whole waves; nothing waits for them
+    const int seg = wave / nstrips, strip = wave - seg * nstrips;
+    const int ks = seg * seg_rows;
+    const int ke = min(ks + seg_rows, nx);
+    const DevSweepLane ln = dev_sweep_lane<K>(strip, ny, int(threadIdx.x & 63));
+    uint32_t seeds[K];  // the dither seed of each step of the launch
+#pragma unroll
+    for (int j = 0; j < K; ++j) seeds[j] = seed + uint32_t(j) * seed_stride;
+    dev_sweep_run<FMT, CT, BC, MOM, K, EACH>(src, dst, ks, ke, nx, ny, L, ln, omega, u0, scale,
+                                             inv_scale, stochastic, seeds, 0, ux_o, uy_o);
+}
+
 // FP64 populations (reference layout) -> stored deviations (any layout).
 template <int FMT>
 __global__ __launch_bounds__(kBlock) void k_dev_encode(const double *__restrict__ f,
@@ -559,7 +580,19 @@ void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc
     X(9, 4, 40, 32, 1024)       \
     X(10, 4, 46, 32, 1024)
 #define LBM_DEV_COUNT_MULTI(id, k, tr, tc, bs) +1
-constexpr int kNumDevMultiShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
+constexpr int kNumDevTileShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
+
+// Shapes of the register sweep on deviations (k_dev_sweep), variants
+// kNumDevTileShapes + id: (steps per launch K, rows per wave segment).  An FP64
+// collide keeps twice the registers per value: K > 4 takes the FP32 collide only.
+#define LBM_DEV_SWEEP_SHAPES(X) \
+    X(0, 4, 32)                 \
+    X(1, 4, 64)                 \
+    X(2, 6, 64)                 \
+    X(3, 8, 64)
+#define LBM_DEV_COUNT_SWEEP(id, k, seg) +1
+constexpr int kNumDevSweepShapes = 0 LBM_DEV_SWEEP_SHAPES(LBM_DEV_COUNT_SWEEP);
+constexpr int kNumDevMultiShapes = kNumDevTileShapes + kNumDevSweepShapes;
 // Four steps on 46 x 32 tiles with 1024 threads: the fastest measured for FP32
 // and FP16 storage (at 8192^2, FP16 stored every 4 steps 230 GLUPS and FP32
 // storage 224 in a same-run comparison).  Its four rings (52x38 .. 46x32 nodes)
@@ -599,16 +632,49 @@ int launch_dev_multi_v(const void *src, void *dst, const DevArgs &a, uint32_t se
     return LBM_OK;
 }
 
+template <int FMT, typename CT, int BC, bool MOM, int K>
+int launch_dev_sweep_v(const void *src, void *dst, const DevArgs &a, uint32_t seed_stride,
+                       bool round_each, int seg_rows, hipStream_t s) {
+    using R = typename Codec<FMT>::raw;
+    if (a.nx < 2 * K || a.ny < 64) return LBM_E_SIZE;
+    if constexpr (sizeof(CT) == 8 && K > 4) {
+        return LBM_E_ARG;
+    } else {
+        const int64_t nstrips = dev_sweep_strips<K>(a.ny), nsegs = (a.nx + seg_rows - 1) / seg_rows;
+        const int64_t nwaves = nstrips * nsegs;
+        const int64_t nblocks = (nwaves + 3) / 4;
+        const int64_t grid = ((nblocks + 7) / 8) * 8;
+        if (grid > 0x7fffffffLL) return LBM_E_SIZE;
+        if (round_each)
+            hipLaunchKernelGGL((k_dev_sweep<FMT, CT, BC, MOM, K, true>), dim3(unsigned(grid)),
+                               dim3(256), 0, s, static_cast<const R *>(src), static_cast<R *>(dst),
+                               a.nx, a.ny, a.L, int(nstrips), seg_rows, int(nwaves), int(nblocks),
+                               CT(a.omega), CT(a.u0), CT(a.scale), CT(1) / CT(a.scale),
+                               a.stochastic, a.seed, seed_stride, a.ux, a.uy);
+        else
+            hipLaunchKernelGGL((k_dev_sweep<FMT, CT, BC, MOM, K, false>), dim3(unsigned(grid)),
+                               dim3(256), 0, s, static_cast<const R *>(src), static_cast<R *>(dst),
+                               a.nx, a.ny, a.L, int(nstrips), seg_rows, int(nwaves), int(nblocks),
+                               CT(a.omega), CT(a.u0), CT(a.scale), CT(1) / CT(a.scale),
+                               a.stochastic, a.seed, seed_stride, a.ux, a.uy);
+        return LBM_OK;
+    }
+}
+
 template <int FMT, typename CT, int BC, bool MOM>
 int launch_dev_multi_bc(const void *src, void *dst, const DevArgs &a, uint32_t stride, bool each,
                         int shape, hipStream_t s) {
 #define LBM_DEV_LAUNCH_MULTI(id, k, tr, tc, bs) \
     case id: return launch_dev_multi_v<FMT, CT, BC, MOM, k, tr, tc, bs>(src, dst, a, stride, each, s);
+#define LBM_DEV_LAUNCH_SWEEP(id, k, seg) \
+    case kNumDevTileShapes + id: return launch_dev_sweep_v<FMT, CT, BC, MOM, k>(src, dst, a, stride, each, seg, s);
     switch (shape) {
         LBM_DEV_MULTI_SHAPES(LBM_DEV_LAUNCH_MULTI)
+        LBM_DEV_SWEEP_SHAPES(LBM_DEV_LAUNCH_SWEEP)
         default: return LBM_E_ARG;
     }
 #undef LBM_DEV_LAUNCH_MULTI
+#undef LBM_DEV_LAUNCH_SWEEP
 }
 
 template <int FMT, typename CT>
@@ -705,12 +771,12 @@ int lbm_dev_decode(const void *d, double *f, int64_t nx, int64_t ny, int layout,
 }
 
 int lbm_dev_step(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                 float omega, int bc, float u0, float scale, int compute, int rounding,
+                 double omega, int bc, double u0, double scale, int compute, int rounding,
                  uint32_t seed, float *ux, float *uy, int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if ((ux == nullptr) != (uy == nullptr)) return LBM_E_NULL;
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
-    if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0f)) return LBM_E_ARG;
+    if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
     if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
     if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;
@@ -735,21 +801,25 @@ int lbm_dev_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny
     const int shape = variant < 0 ? kDefaultDevMultiShape : variant;
 #define LBM_DEV_INFO_MULTI(id, k, tr, tc, bs) \
     case id: *steps = k; *min_nx = tr + 2 * k; *min_ny = tc + 2 * k; return LBM_OK;
+#define LBM_DEV_INFO_SWEEP(id, k, seg) \
+    case kNumDevTileShapes + id: *steps = k; *min_nx = 2 * k; *min_ny = 64; return LBM_OK;
     switch (shape) {
         LBM_DEV_MULTI_SHAPES(LBM_DEV_INFO_MULTI)
+        LBM_DEV_SWEEP_SHAPES(LBM_DEV_INFO_SWEEP)
         default: return LBM_E_ARG;
     }
 #undef LBM_DEV_INFO_MULTI
+#undef LBM_DEV_INFO_SWEEP
 }
 
 int lbm_dev_step_multi(const void *src, void *dst, int fmt, int64_t nx, int64_t ny, int layout,
-                       float omega, int bc, float u0, float scale, int compute, int rounding,
+                       double omega, int bc, double u0, double scale, int compute, int rounding,
                        uint32_t seed, uint32_t seed_stride, int round_each, float *ux, float *uy,
                        int variant, void *stream) {
     if (!src || !dst) return LBM_E_NULL;
     if ((ux == nullptr) != (uy == nullptr)) return LBM_E_NULL;
     if (!valid_grid(nx, ny) || nx > INT32_MAX || ny > INT32_MAX) return LBM_E_SIZE;
-    if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0f)) return LBM_E_ARG;
+    if (!valid_fmt(fmt) || !valid_layout(layout) || !(scale > 0.0)) return LBM_E_ARG;
     if (bc != LBM_BC_PERIODIC && bc != LBM_BC_CAVITY) return LBM_E_ARG;
     if (compute != LBM_COMPUTE_F32 && compute != LBM_COMPUTE_F64) return LBM_E_ARG;
     if (rounding != LBM_ROUND_NEAREST && rounding != LBM_ROUND_STOCHASTIC) return LBM_E_ARG;

@@ -134,7 +134,9 @@ class Lattice:
         if variant is None:
             return
         if variant < 0:
-            variant = nat.load().lbm_step_multi_default_variant()
+            variant = nat.load().lbm_step_multi_default_for(self.nx, self.ny)
+            if variant < 0:
+                variant = nat.load().lbm_step_multi_default_variant()
         k, mnx, mny = kernels.step_multi_info(variant)
         if self.nx >= mnx and self.ny >= mny:
             self.multi_variant, self.multi_k = int(variant), k

@@ -84,8 +84,8 @@ class DeviationLattice:
         self._e = torch.empty_like(self.d)
         self.multi_variant, self.multi_k, self.round_each = None, 1, True
         self.store_every = int(store_every)
-        if self.store_every not in (1, 2, 3, 4):
-            raise ValueError("store_every must be 1, 2, 3 or 4")
+        if self.store_every not in (1, 2, 3, 4, 6, 8):
+            raise ValueError("store_every must be 1, 2, 3, 4, 6 or 8")
         auto = variant == "auto"
         if auto:
             variant = self._tune()
@@ -101,8 +101,9 @@ class DeviationLattice:
         self.time = 0
         self.init_equilibrium()
 
-    # shapes tried first for each K when storing every K steps
-    _STORE_EVERY_SHAPES = {2: 1, 3: 3, 4: 10}
+    # shapes tried first for each K when storing every K steps (6 and 8: the
+    # register sweeps, variants 13 and 14)
+    _STORE_EVERY_SHAPES = {2: 1, 3: 3, 4: 10, 6: 13, 8: 14}
 
     def _set_store_every(self, K, tune):
         """Store every K steps: runs of K steps through one multi-step launch
@@ -156,11 +157,17 @@ class DeviationLattice:
         from .autotune import best_variant
         n_var = nat.load().lbm_dev_multi_variant_count()
         info = {v: multi_info(v) for v in range(n_var)}
-        cands = ["s%d" % single] + ["m%d" % v for v, (k, a, b) in info.items()
-                                    if self.nx >= a and self.ny >= b]
+        cands = ["s%d" % single]
+        a, b = self.d.zero_(), self._e  # scratch: the state is initialised after tuning
+        for v, (k, mnx, mny) in info.items():
+            if self.nx >= mnx and self.ny >= mny:
+                try:  # the library refuses shapes this compute type cannot hold
+                    self._call_multi(a, b, v)
+                    cands.append("m%d" % v)
+                except ValueError:
+                    pass
 
         def make_launch(nx, ny):
-            a, b = self.d.zero_(), self._e  # scratch: the state is initialised after tuning
 
             def launch(c):
                 if c[0] == "s":

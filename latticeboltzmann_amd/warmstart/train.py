@@ -24,7 +24,7 @@ import time
 import numpy as np
 import torch
 
-from .features import in_channels
+from .features import RE_RANGE, U_RANGE, in_channels
 from .network import Surrogate
 from .solve import Predictor
 
@@ -114,9 +114,12 @@ def large_scale_error(pred, X, Y, batch, block=32):
 
 
 def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, widths=(32, 64, 128),
-          mid=128, time_limit=None, log=None, octaves=0, init=None, scales=(), ema=0.0):
+          mid=128, time_limit=None, log=None, octaves=0, init=None, scales=(), ema=0.0,
+          re_range=RE_RANGE, u_range=U_RANGE):
     """Train from scratch, or continue from the predictor saved at ``init``
-    (same architecture, inputs and data split).
+    (same architecture and data split; a network trained for other input
+    ranges is a starting point only).  ``re_range``/``u_range`` normalize the
+    inputs and are the ranges the saved predictor accepts.
 
     With ``ema`` > 0 the saved network is the exponential moving average of
     the weights (decay ``ema`` per step).  At every log point (10 epochs) the
@@ -124,8 +127,13 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
     velocity error, and the best one is kept."""
     params, targets = load_dataset(paths)
     S = len(params)
+    lo_re, hi_re = min(p[0] for p in params), max(p[0] for p in params)
+    lo_u, hi_u = min(p[1] for p in params), max(p[1] for p in params)
+    if lo_re < re_range[0] or hi_re > re_range[1] or lo_u < u_range[0] or hi_u > u_range[1]:
+        raise ValueError("training data spans Re %.4g..%.4g, U %.4g..%.4g: outside the ranges %s, %s"
+                         % (lo_re, hi_re, lo_u, hi_u, re_range, u_range))
     pred = Predictor(Surrogate(in_ch=in_channels(octaves), widths=widths, mid=mid, seed=seed),
-                     octaves=octaves)
+                     re_range=re_range, u_range=u_range, octaves=octaves)
     net = pred.net
     X = pred.inputs(params)
     rng = np.random.default_rng(seed)
@@ -216,11 +224,14 @@ def main(argv=None):
     ap.add_argument("--ema", type=float, default=0.0,
                     help="keep an exponential moving average of the weights with this decay per step")
     ap.add_argument("--time-limit", type=float, default=None)
+    ap.add_argument("--re-range", type=float, nargs=2, default=RE_RANGE)
+    ap.add_argument("--u-range", type=float, nargs=2, default=U_RANGE)
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     train(a.data, a.epochs, a.batch, a.lr, a.seed, a.val_frac, a.out, time_limit=a.time_limit,
           log=lambda s: print(s, flush=True), octaves=a.octaves, init=a.init,
-          scales=[(int(v.split(":")[0]), float(v.split(":")[1])) for v in a.scale_loss], ema=a.ema)
+          scales=[(int(v.split(":")[0]), float(v.split(":")[1])) for v in a.scale_loss], ema=a.ema,
+          re_range=tuple(a.re_range), u_range=tuple(a.u_range))
 
 
 if __name__ == "__main__":

@@ -111,9 +111,10 @@ def test_dropin_imports_as_top_level_pylb():
 
 
 def test_multi_step_shape_tables_without_gpu():
-    # every multi-step shape reports 2..4 steps and a smallest grid of at
-    # least its tile plus the halo; the defaults are shapes of the tables, and
-    # each store-every-K shape of DeviationLattice steps K times per launch
+    # every multi-step shape reports 2..8 steps and a smallest grid of at
+    # least its tile plus the halo (the row sweeps: 2K rows, one wave of 64
+    # columns); the defaults are shapes of the tables, and each store-every-K
+    # shape of DeviationLattice steps K times per launch
     from latticeboltzmann_amd import kernels
     from latticeboltzmann_amd.precision import DeviationLattice, multi_info
     from latticeboltzmann_amd import _native
@@ -124,8 +125,18 @@ def test_multi_step_shape_tables_without_gpu():
             (lib.lbm_dev_multi_variant_count(), multi_info, lib.lbm_dev_multi_default_variant())):
         assert 0 <= default < count
         assert info(-1) == info(default)
+    # the size-dependent default: the tile below 4096^2 nodes, a sweep from there
+    small, large = lib.lbm_step_multi_default_for(2048, 2048), lib.lbm_step_multi_default_for(8192, 8192)
+    assert small == lib.lbm_step_multi_default_variant()
+    assert 0 <= large < lib.lbm_step_multi_variant_count() and large != small
+    assert kernels.step_multi_info(large)[2] == 64  # register sweeps need 64 columns
+    assert lib.lbm_step_multi_default_for(0, 5) == -2
+    for count, info, default in (
+            (lib.lbm_step_multi_variant_count(), kernels.step_multi_info,
+             lib.lbm_step_multi_default_variant()),
+            (lib.lbm_dev_multi_variant_count(), multi_info, lib.lbm_dev_multi_default_variant())):
         for v in range(count):
             K, mnx, mny = info(v)
-            assert 2 <= K <= 4 and mnx >= 2 * K + 1 and mny >= 2 * K + 1, (v, K, mnx, mny)
+            assert 2 <= K <= 8 and mnx >= 2 * K and mny >= 2 * K + 1, (v, K, mnx, mny)
     for K, shape in DeviationLattice._STORE_EVERY_SHAPES.items():
         assert multi_info(shape)[0] == K, (K, shape)

@@ -121,3 +121,28 @@ def test_slow_decay_16bit_storage_with_fp64_compute(gpu, fmt):
     print(fmt, errs)
     assert errs["f64"].max() < 2e-2, errs
     assert errs["f64"].max() < errs["f32"].max(), errs
+
+
+@pytest.mark.parametrize("compute", ["f32", "f64"])
+@pytest.mark.parametrize("fmt", ["f32", "f16"])
+def test_batch_multi_step_equals_single_steps(gpu, compute, fmt):
+    # four steps per launch (the register sweep over a work list of all
+    # instances) against one step per launch, stochastic rounding included:
+    # eager steps with a remainder, then graph-replayed chunks
+    from latticeboltzmann_amd.batch import BatchedLattices
+    kw = dict(fmt=fmt, compute=compute, rounding="stochastic", seed=9, variant=1)
+    a = BatchedLattices(MIXED, steps_per_launch=4, **kw)
+    b = BatchedLattices(MIXED, steps_per_launch=1, **kw)
+    a.step(13)
+    b.step(13)
+    assert torch.equal(a.a, b.a)
+    ta, amp_a = a.run(40, sample_every=20, graph=True)
+    tb, amp_b = b.run(40, sample_every=20, graph=True)
+    assert np.array_equal(ta, tb)
+    assert torch.equal(a.a, b.a)
+    assert np.array_equal(amp_a, amp_b)
+    c = BatchedLattices(MIXED, steps_per_launch=2, **kw)
+    c.step(13)
+    b2 = BatchedLattices(MIXED, steps_per_launch=1, **kw)
+    b2.step(13)
+    assert torch.equal(c.a, b2.a)

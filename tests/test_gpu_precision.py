@@ -89,8 +89,8 @@ def test_one_step_against_fp64_oracle(gpu, fmt, bc, shape, rounding):
 def test_one_step_fp64_collide_against_oracle(gpu, fmt, rounding):
     # FP64 collide, one rounding to the storage format: within the
     # compensated-rounding bounds of the FP64 oracle step, in every launch shape
-    # (omega and u0 cross the C ABI as FP32 values, which the FP64 collide
-    # uses exactly, so the oracle gets the same FP32-representable values)
+    # (omega and u0 are FP32-representable, so every compute type and the
+    # oracle use the same values)
     from latticeboltzmann_amd import _native
     from latticeboltzmann_amd.precision import DeviationLattice
     w = o.weights()[:, None, None]
@@ -206,7 +206,7 @@ def test_shear_wave_viscosity_fp16(gpu):
 @pytest.mark.parametrize("fmt", lp.FORMATS)
 @pytest.mark.parametrize("compute", ["f32", "f64"])
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
-@pytest.mark.parametrize("K", [2, 3, 4])
+@pytest.mark.parametrize("K", [2, 3, 4, 6, 8])
 def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
     # storing every K steps: K unrounded steps on chip, then one compensated
     # rounding, so the stored state lies within the one-step rounding bounds of
@@ -215,6 +215,8 @@ def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
     w = o.weights()[:, None, None]
     s = SCALES[fmt]
     om, u0 = float(np.float32(1.3)), float(np.float32(0.1))
+    if compute == "f64" and K > 4:
+        pytest.skip("store every 6 or 8 steps: FP32 collide only (register sweep)")
     shape = (24, 80)
     f = populations(*shape)
     for bc in ("cavity", "periodic"):

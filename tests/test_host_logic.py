@@ -27,13 +27,13 @@ def test_instance_and_tile_records_match_the_c_structs():
 
     class Inst(ctypes.Structure):
         _fields_ = [("offset", ctypes.c_int64), ("nx", ctypes.c_int32), ("ny", ctypes.c_int32),
-                    ("omega", ctypes.c_float), ("u0", ctypes.c_float), ("bc", ctypes.c_int32),
+                    ("omega", ctypes.c_double), ("u0", ctypes.c_double), ("bc", ctypes.c_int32),
                     ("dir", ctypes.c_int32)]
 
     class Tile(ctypes.Structure):
         _fields_ = [("inst", ctypes.c_int32), ("first", ctypes.c_int32)]
 
-    assert INST_DTYPE.itemsize == ctypes.sizeof(Inst) == 32
+    assert INST_DTYPE.itemsize == ctypes.sizeof(Inst) == 40
     assert TILE_DTYPE.itemsize == ctypes.sizeof(Tile) == 8
     for name, _ in Inst._fields_:
         assert INST_DTYPE.fields[name][1] == getattr(Inst, name).offset
