@@ -20,14 +20,21 @@ one persistent graph-captured launch per step) with its fitted-viscosity
 errors; and the warm-start time to steady state against the cold start on
 held-out 256 x 256 cavities.
 
-``roofline``: algorithmic bytes per lattice update = 2 * 9 * 8 (read and
-write all nine FP64 populations once) = 144 B; achieved = 144 B * lattice
-updates per launch (K per node for a K-step launch) / average launch
-duration; peak = 8 TB/s (MI355X HBM3E).  A K-step launch moves about 1/K of
-its algorithmic bytes through HBM, so ``frac`` can exceed 1: ``traffic`` is
-the HBM bytes per launch from rocprofv3 FETCH_SIZE + WRITE_SIZE counters (a
-committed summary under profiles/, null if absent), and
-``extra.hbm_gbs_from_counters`` the HBM rate it implies.
+``roofline``: the dominant kernel is the K-step launch (K steps per launch,
+the earlier steps kept on chip), which moves about 1/K of a single pass's
+bytes and is limited by its FP64 arithmetic instead.  Its hardware counters
+(a committed summary under profiles/, taken with the same build and kernel
+variant) show the VALU as the busiest unit, so the roofline is the VALU issue
+rate: peak = 256 CUs x 64 lanes x 2.4 GHz = 39.3 T lane-ops/s (MI355X's
+published 78.6 TFLOP/s FP64 vector rate, which counts an FMA as two flops);
+achieved = the algorithmic FP64 work (128 FP64 VALU instructions per lattice
+update: the opt1 node arithmetic without contraction, two IEEE divides,
+counted in the single-step kernel's code) x lattice updates per launch /
+average launch time, measured with HIP events on the launch stream.
+``traffic`` is the HBM bytes per launch from FETCH_SIZE + WRITE_SIZE.
+``extra.roofline_detail`` gives the counter view at the live launch time: the
+fraction of VALU issue slots used (recomputed halo and index arithmetic
+included) and the HBM rate.
 
 ``cpu_baseline``: the reference's CPU paths restated by the oracle, timed on
 one host core on a bounded sample of the same 2048^2 cavity: ``value`` is the
@@ -50,6 +57,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LUP = {"f64": 144, "f32": 72, "bf16": 36, "f16": 36}
+# one VALU operation per lane per clock: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+# FP64 VALU instructions of one lattice update of the bit-exact opt1 node
+# arithmetic (pull, moments, two IEEE divides, nine equilibria, relaxation; no
+# fused multiply-adds), counted in the compiled single-step kernel
+FP64_OPS_PER_LUP = 128
 
 
 def parse():
@@ -159,9 +172,9 @@ def copy_bandwidth(nbytes):
     return 2 * nbytes / sec / 1e9
 
 
-def profiled_traffic(name, variant=None):
-    """HBM bytes per launch from a committed counter summary (per variant when
-    the summary holds several), or None."""
+def kernel_profile(name, variant=None):
+    """A committed counter summary (profiles/<name>), per variant when it holds
+    several, or None."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as fh:
@@ -169,8 +182,8 @@ def profiled_traffic(name, variant=None):
     except (OSError, ValueError):
         return None
     if variant is not None and "per_variant" in j:
-        return j["per_variant"].get(str(variant), {}).get("hbm_bytes_per_launch")
-    return j.get("hbm_bytes_per_launch")
+        return j["per_variant"].get(str(variant))
+    return j
 
 
 def _cpu_leg(args):
@@ -244,16 +257,25 @@ def batch_sweep(steps=2000):
     return out
 
 
-def dumped_runs(n, omega, steps, plain_s, freqs=(10000, 1000)):
+def dumped_runs(n, omega, steps=20000, freqs=(10000, 1000)):
     """The cavity run again with the reference's velocity dumps (ux_i.npy,
     uy_i.npy every dump_freq steps and at the last step) written to a
-    temporary directory; wall time against the plain run of the same steps."""
+    temporary directory; wall time against a plain run of the same steps
+    timed the same way (the workload's 20000 steps, whatever --steps is)."""
     import shutil
     import tempfile
     import torch
     from latticeboltzmann_amd.dumps import run_cavity
     from latticeboltzmann_amd.lattice import Lattice
-    out = {}
+    lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant="auto")
+    lat.step(20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lat.step(steps)
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    del lat
+    out = {"steps": steps, "plain_seconds": round(plain_s, 4)}
     for freq in freqs:
         lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant="auto")
         lat.step(20)
@@ -289,7 +311,7 @@ def ghost_flow(n=4096, steps=200):
     ok = bool(torch.isfinite(lat.f).all())
     del lat
     torch.cuda.empty_cache()
-    return {"nodes": nodes, "steps": steps, "mlups": round(m, 1), "achieved_gbs": round(gbs, 1),
+    return {"nodes": nodes, "steps": steps, "mlups": round(m, 1), "hbm_gbs_algorithmic": round(gbs, 1),
             "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4), "finite": ok}
 
 
@@ -370,34 +392,54 @@ def main():
     sec = timed_steps(lat, args.steps, args.warmup)
     mlups = n * n * args.steps / sec / 1e6
     launch_s = sec / launches
-    # algorithmic bytes of one launch: 144 B per lattice update, K updates per
-    # node for a K-step launch
-    alg_per_launch = BYTES_PER_LUP["f64"] * n * n * args.steps / launches
-    achieved = alg_per_launch / launch_s / 1e9
+    lups_per_launch = n * n * args.steps / launches
     ok = bool(torch.isfinite(lat.f).all())
     del lat
     torch.cuda.empty_cache()
     if multi_var is not None:
-        dominant = "k_stepk (%d steps per launch, variant %d)" % (multi_k, multi_var)
-        traffic = profiled_traffic("step_multi_f64_cavity_%d_traffic.json" % n, multi_var)
+        dominant = "%d steps per launch, multi-step variant %d" % (multi_k, multi_var)
+        prof = kernel_profile("roofline_f64_cavity_%d.json" % n, "multi%d" % multi_var)
     else:
-        dominant = "k_step"
-        traffic = profiled_traffic("step_f64_cavity_%d_traffic.json" % n)
+        dominant = "single step, variant %d" % lat_variant
+        prof = kernel_profile("roofline_f64_cavity_%d.json" % n, "single%d" % lat_variant)
+    alg_tops = FP64_OPS_PER_LUP * lups_per_launch / launch_s / 1e12
+    traffic = prof.get("hbm_bytes_per_launch") if prof else None
+    roofline = {"bound": "valu", "achieved": round(alg_tops, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "T VALU lane-op/s", "frac": round(alg_tops / VALU_PEAK_TOPS, 4),
+                "traffic": traffic}
+    detail = {"kernel": prof.get("kernel") if prof else None,
+              "fp64_ops_per_lattice_update": FP64_OPS_PER_LUP,
+              "lattice_updates_per_launch": int(lups_per_launch),
+              "launch_us": round(launch_s * 1e6, 2),
+              # HBM bytes a launch must move: one read and one write of every
+              # population per launch, however many steps it makes
+              "compulsory_bytes_per_launch": BYTES_PER_LUP["f64"] * n * n,
+              "compulsory_gbs": round(BYTES_PER_LUP["f64"] * n * n / launch_s / 1e9, 1),
+              # what a single pass per step would have to move at this rate
+              "single_pass_equivalent_gbs": round(
+                  BYTES_PER_LUP["f64"] * lups_per_launch / launch_s / 1e9, 1)}
+    if prof:
+        detail["valu_issue_frac_from_counters"] = round(
+            prof["valu_wave_insts_per_launch"] * 64 / launch_s / 1e12 / VALU_PEAK_TOPS, 4)
+        detail["valu_lane_ops_per_lattice_update_from_counters"] = round(
+            prof["valu_wave_insts_per_launch"] * 64 / lups_per_launch, 1)
+        detail["hbm_gbs_from_counters"] = round(traffic / launch_s / 1e9, 1)
+        detail["hbm_frac_from_counters"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+        detail["profiled_launch_us"] = prof.get("kernel_trace_avg_us")
 
     extra = {"finite_state": ok, "omega": omega, "variant": lat_variant,
              "multi_variant": multi_var, "steps_per_launch": multi_k, "dominant_kernel": dominant,
-             "launches": launches, "algorithmic_bytes_per_launch": int(alg_per_launch)}
-    if traffic:
-        extra["hbm_gbs_from_counters"] = round(traffic / launch_s / 1e9, 1)
-        extra["hbm_frac_from_counters"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
+             "launches": launches, "roofline_detail": detail}
     if multi_var is not None and not args.no_extra:
         lat = Lattice(n, n, omega, "cavity", 0.1, torch.float64, variant=args.variant,
                       multi_step=False)
         s1 = timed_steps(lat, 2000, 50)
         del lat
         torch.cuda.empty_cache()
-        extra["single_step_%d" % n] = {"mlups": round(n * n * 2000 / s1 / 1e6, 1),
-                                       "achieved_gbs": round(144 * n * n * 2000 / s1 / 1e9, 1)}
+        m1 = n * n * 2000 / s1 / 1e6
+        extra["single_step_%d" % n] = {"mlups": round(m1, 1),
+                                       "hbm_gbs_algorithmic": round(144 * m1 / 1e3, 1),
+                                       "frac_of_8tbs": round(144 * m1 / 1e3 / HBM_PEAK_GBS, 4)}
     if not args.no_extra:
         from latticeboltzmann_amd.precision import DeviationLattice
         big = 8192
@@ -412,10 +454,11 @@ def main():
             steps = 200
             s = timed_steps(lat, steps, 20)
             m = big * big * steps / s / 1e6
-            gbs = BYTES_PER_LUP[name] * m * 1e6 / 1e9
+            k = lat.multi_k
             legs[name] = {"mlups": round(m, 1), "steps": steps, "variant": lat.variant,
-                          "steps_per_launch": lat.multi_k,
-                          "achieved_gbs": round(gbs, 1), "frac_of_8tbs": round(gbs / HBM_PEAK_GBS, 4)}
+                          "multi_variant": lat.multi_variant, "steps_per_launch": k,
+                          # one read and one write of every population per launch
+                          "compulsory_gbs": round(BYTES_PER_LUP[name] / k * m / 1e3, 1)}
             del lat
             torch.cuda.empty_cache()
         lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant,
@@ -424,7 +467,7 @@ def main():
         del lat
         torch.cuda.empty_cache()
         m = big * big * 200 / s / 1e6
-        legs["f64_single_step"] = {"mlups": round(m, 1), "achieved_gbs": round(144 * m / 1e3, 1),
+        legs["f64_single_step"] = {"mlups": round(m, 1), "hbm_gbs_algorithmic": round(144 * m / 1e3, 1),
                                    "frac_of_8tbs": round(144 * m / 1e3 / HBM_PEAK_GBS, 4)}
         # FP16 storage against FP64 with temporal blocking (the fastest FP64
         # form) and against the plain single-pass FP64 kernel
@@ -437,7 +480,7 @@ def main():
             me = big * big * 48 * every / s / 1e6
             key = "f16_store_every_%d" % every
             legs[key] = {"mlups": round(me, 1), "steps": 48 * every, "steps_per_launch": every,
-                         "achieved_gbs": round(36 * me / 1e3, 1)}
+                         "compulsory_gbs": round(36 / every * me / 1e3, 1)}
             legs[key + "_over_f64_mlups"] = round(me / legs["f64"]["mlups"], 3)
         legs["f16_over_f64_mlups"] = round(legs["f16"]["mlups"] / legs["f64"]["mlups"], 3)
         legs["f16_over_f64_single_step_mlups"] = round(legs["f16"]["mlups"] / m, 3)
@@ -446,10 +489,11 @@ def main():
         extra["device_copy_gbs_same_bytes"] = round(copy_bandwidth(nbytes), 1)
         ceiling = copy_ceiling(nbytes)
         extra["copy_ceiling_gbs"] = round(ceiling, 1)
-        extra["f64_8192_over_copy_ceiling"] = round(legs["f64"]["achieved_gbs"] / ceiling, 3)
+        extra["f64_8192_single_step_over_copy_ceiling"] = round(
+            legs["f64_single_step"]["hbm_gbs_algorithmic"] / ceiling, 3)
         extra["ghost_sliding_lid_4098"] = ghost_flow()
     if not args.no_dumps:
-        extra["dumps_%d" % n] = dumped_runs(n, omega, args.steps, sec)
+        extra["dumps_%d" % n] = dumped_runs(n, omega)
     if not args.no_extra:
         extra["precision_%d" % n] = precision_check(n, omega, args.steps)
     if not args.no_batch:
@@ -473,9 +517,7 @@ def main():
         "config": {"workload": "lid-driven cavity %dx%d FP64 D2Q9 BGK, Re=1000, U=0.1, "
                                "omega=%.6f, fused pull stream-collide" % (n, n, omega),
                    "nx": n, "ny": n, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic},
+        "roofline": roofline,
         "cpu_baseline": None,
         "extra": extra,
     }

@@ -184,10 +184,12 @@ SMALL = [  # batched instance sides are multiples of 64
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
 def test_batched_instances_against_fp64_oracle(gpu, compute, rounding):
     # one batched step of each instance against the FP64 oracle step of its
-    # decoded stored state, with the one-step bounds of standalone storage:
-    # FP32 collide into FP32 storage within one FP32 ulp (+ collide rounding);
-    # an FP64 collide rounded once (compensated) within 4 (nearest) or 6
-    # (stochastic) ulps of each node's largest value
+    # decoded stored state.  FP32 collide into FP32 storage: within one FP32
+    # ulp of the value plus 4 ulps of the node's sum of magnitudes (the density
+    # and momentum sums round at the scale of sum_i |x_i|, and every relaxed
+    # population inherits omega w_i times that error); an FP64 collide rounded
+    # once (compensated) within 4 (nearest) or 6 (stochastic) ulps of each
+    # node's largest value
     from latticeboltzmann_amd.batch import BatchedLattices
     bat = BatchedLattices(SMALL, fmt="f32", compute=compute, rounding=rounding, variant=0)
     rng = np.random.default_rng(11)
@@ -211,7 +213,7 @@ def test_batched_instances_against_fp64_oracle(gpu, compute, rounding):
         got = bat.populations(i)
         x_exp, x_got = expect - w, got - w
         if compute == "f32":
-            tol = lp.ulp(x_exp, "f32") + 1e-8
+            tol = lp.ulp(x_exp, "f32") + 4 * lp.ulp(np.abs(x_exp).sum(0), "f32")[None]
         else:
             big = lp.ulp(np.abs(x_exp).max(0), "f32")
             tol = (4 if rounding == "nearest" else 6) * big[None] + 1e-12
