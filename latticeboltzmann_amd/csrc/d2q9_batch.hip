@@ -211,20 +211,6 @@ inline int batch_tile_nodes(int shape) {
     return 0;
 }
 
-// Compute units of the current device (cached per device ordinal).
-inline int device_cus() {
-    static int cus[64] = {0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-    if (cus[dev] == 0) {
-        int n = 0;
-        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0)
-            n = 256;
-        cus[dev] = n;
-    }
-    return cus[dev];
-}
-
 struct BatchArgs {
     const void *src;
     void *dst;

@@ -767,7 +767,8 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
 constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
 
 // Shapes of the register sweep (k_sweep), variants kNumTileShapes + id:
-// (steps per launch K, rows per wave segment, ping-pong buffers, prefetch).
+// (steps per launch K, rows per wave segment (0: balanced, balanced_seg_rows),
+// ping-pong buffers, prefetch).
 #define LBM_SWEEP_SHAPES(X)    \
     X(0, 4, 16, true, true)    \
     X(1, 4, 32, true, true)    \
@@ -778,7 +779,9 @@ constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
     X(6, 5, 64, false, false)  \
     X(7, 5, 64, false, true)   \
     X(8, 6, 64, false, false)  \
-    X(9, 3, 32, false, true)
+    X(9, 3, 32, false, true)   \
+    X(10, 5, 0, true, false)   \
+    X(11, 4, 0, true, true)
 #define LBM_COUNT_SWEEP(id, k, seg, pp, pf) +1
 constexpr int kNumSweepShapes = 0 LBM_SWEEP_SHAPES(LBM_COUNT_SWEEP);
 constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
@@ -787,11 +790,11 @@ constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
 // per CU).  Its four rings (26x38, 24x36, 22x34, 20x32 nodes) take two passes
 // of the block each, with few idle lanes in the second.
 constexpr int kDefaultMultiShape = 18;
-// From 4096^2 nodes on, the five-step sweep with 64-row segments and ping-pong
-// row buffers: 131-141 GLUPS at 8192^2 against 115-121 for the tile.  On
-// smaller grids its waves (one per 58 columns x 64 rows) are too few to hide
-// memory latency (83 GLUPS at 2048^2).
-constexpr int kLargeGridMultiShape = kNumTileShapes + 5;
+// From 4096^2 nodes on, the five-step sweep with ping-pong row buffers and
+// balanced segments: 124 GLUPS at 4096^2 and 139 at 8192^2 against 114 and
+// 122 for the tile.  At 2048^2 the two tie (105-106); on smaller grids the
+// sweep's waves are too few to hide memory latency (64 against 94 GLUPS at 1024^2).
+constexpr int kLargeGridMultiShape = kNumTileShapes + 10;
 constexpr int64_t kLargeGridNodes = int64_t(4096) * 4096;
 
 inline int default_multi_shape(int64_t nx, int64_t ny) {
@@ -817,7 +820,13 @@ int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int s
                    T u0, T *rho, T *ux, T *uy, hipStream_t s) {
     constexpr int VW = 64 - 2 * (K - 1);
     if (nx < 2 * K || ny < 64) return LBM_E_SIZE;
-    const int64_t nstrips = (ny + VW - 1) / VW, nsegs = (nx + seg_rows - 1) / seg_rows;
+    const int64_t nstrips = (ny + VW - 1) / VW;
+    if (seg_rows == 0) {
+        static std::atomic<int> per_cu[64];
+        seg_rows = balanced_seg_rows(nx, nstrips, resident_waves(k_sweep<T, BC, MOM, K, PP, PF>, per_cu),
+                                     2 * (K - 1));
+    }
+    const int64_t nsegs = (nx + seg_rows - 1) / seg_rows;
     const int64_t nwaves = nstrips * nsegs;
     const int64_t nblocks = (nwaves + 3) / 4;
     const int64_t grid = ((nblocks + 7) / 8) * 8;

@@ -583,13 +583,16 @@ void launch_dev_compute(const void *src, void *dst, int fmt, int compute, int bc
 constexpr int kNumDevTileShapes = 0 LBM_DEV_MULTI_SHAPES(LBM_DEV_COUNT_MULTI);
 
 // Shapes of the register sweep on deviations (k_dev_sweep), variants
-// kNumDevTileShapes + id: (steps per launch K, rows per wave segment).  An FP64
-// collide keeps twice the registers per value: K > 4 takes the FP32 collide only.
+// kNumDevTileShapes + id: (steps per launch K, rows per wave segment, 0 for
+// balanced segments as balanced_seg_rows).  An FP64 collide keeps twice the
+// registers per value: K > 4 takes the FP32 collide only.
 #define LBM_DEV_SWEEP_SHAPES(X) \
     X(0, 4, 32)                 \
     X(1, 4, 64)                 \
     X(2, 6, 64)                 \
-    X(3, 8, 64)
+    X(3, 8, 64)                 \
+    X(4, 4, 0)                  \
+    X(5, 6, 0)
 #define LBM_DEV_COUNT_SWEEP(id, k, seg) +1
 constexpr int kNumDevSweepShapes = 0 LBM_DEV_SWEEP_SHAPES(LBM_DEV_COUNT_SWEEP);
 constexpr int kNumDevMultiShapes = kNumDevTileShapes + kNumDevSweepShapes;
@@ -640,7 +643,15 @@ int launch_dev_sweep_v(const void *src, void *dst, const DevArgs &a, uint32_t se
     if constexpr (sizeof(CT) == 8 && K > 4) {
         return LBM_E_ARG;
     } else {
-        const int64_t nstrips = dev_sweep_strips<K>(a.ny), nsegs = (a.nx + seg_rows - 1) / seg_rows;
+        const int64_t nstrips = dev_sweep_strips<K>(a.ny);
+        if (seg_rows == 0) {  // balanced segments (balanced_seg_rows)
+            static std::atomic<int> per_cu_each[64], per_cu_once[64];
+            const int64_t resident =
+                round_each ? resident_waves(k_dev_sweep<FMT, CT, BC, MOM, K, true>, per_cu_each)
+                           : resident_waves(k_dev_sweep<FMT, CT, BC, MOM, K, false>, per_cu_once);
+            seg_rows = balanced_seg_rows(a.nx, nstrips, resident, 2 * (K - 1));
+        }
+        const int64_t nsegs = (a.nx + seg_rows - 1) / seg_rows;
         const int64_t nwaves = nstrips * nsegs;
         const int64_t nblocks = (nwaves + 3) / 4;
         const int64_t grid = ((nblocks + 7) / 8) * 8;

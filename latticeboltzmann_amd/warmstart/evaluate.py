@@ -43,7 +43,7 @@ class _Fixed:
     def __init__(self, q, pool=2):
         self.q, self.pool = q, pool
 
-    def predict(self, re, u):
+    def predict(self, re, u, **kw):
         return self.q
 
 

@@ -27,6 +27,11 @@ DEFAULT_WEIGHTS = os.path.join(_HERE, "surrogate_weights.npz")
 # and 1024, where BGK stays stable up to Re 5000); load_default() routes each
 # cavity to the network whose ranges cover it
 HIGH_RE_WEIGHTS = os.path.join(_HERE, "surrogate_weights_high_re.npz")
+# the lower Reynolds range again, trained on cavities with N of 512 and 1024
+# (the steady fields of the larger lattices differ from the 256^2 ones at the
+# level of the network's error), for lattices from 384 nodes on
+LARGE_N_WEIGHTS = os.path.join(_HERE, "surrogate_weights_large_n.npz")
+ANY_N = (1, 1 << 30)
 
 
 class OutOfRange(WarmStartUnavailable, ValueError):
@@ -36,17 +41,20 @@ class OutOfRange(WarmStartUnavailable, ValueError):
 class Predictor:
     """A trained surrogate plus the normalization it was trained with."""
 
-    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, pool=2, octaves=0):
+    def __init__(self, net, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, pool=2, octaves=0,
+                 n_range=ANY_N):
         self.net, self.grid, self.pool, self.octaves = net, int(grid), int(pool), int(octaves)
         self.re_range, self.u_range = tuple(re_range), tuple(u_range)
+        self.n_range = tuple(int(x) for x in n_range)
         if net.in_ch != in_channels(self.octaves):
             raise ValueError("network takes %d input channels, features give %d"
                              % (net.in_ch, in_channels(self.octaves)))
 
     @classmethod
     def from_state(cls, state, grid=GRID, re_range=RE_RANGE, u_range=U_RANGE, device=None, pool=2,
-                   octaves=0):
-        return cls(Surrogate.from_state(state, device=device), grid, re_range, u_range, pool, octaves)
+                   octaves=0, n_range=ANY_N):
+        return cls(Surrogate.from_state(state, device=device), grid, re_range, u_range, pool, octaves,
+                   n_range)
 
     @classmethod
     def load(cls, path=DEFAULT_WEIGHTS, device=None):
@@ -54,12 +62,12 @@ class Predictor:
             state = {k: z[k] for k in z.files}
         return cls.from_state(state, int(state["grid"]), tuple(state["re_range"]),
                               tuple(state["u_range"]), device, int(state.get("pool", 2)),
-                              int(state.get("octaves", 0)))
+                              int(state.get("octaves", 0)), tuple(state.get("n_range", ANY_N)))
 
     def save(self, path):
         state = self.net.state_dict()
         state.update(grid=self.grid, re_range=np.array(self.re_range), u_range=np.array(self.u_range),
-                     pool=self.pool, octaves=self.octaves)
+                     pool=self.pool, octaves=self.octaves, n_range=np.array(self.n_range))
         np.savez(path, **state)
 
     def inputs(self, params):
@@ -68,21 +76,24 @@ class Predictor:
                       for re, u in params])
         return torch.as_tensor(x).to(self.net.device)
 
-    def covers(self, re, u):
-        """True when (Re, U) lies in the ranges the network was trained on."""
+    def covers(self, re, u, n=None):
+        """True when (Re, U) lies in the ranges the network was trained on (and
+        the lattice size n, when given, in the sizes it is meant for)."""
         return (self.re_range[0] <= re <= self.re_range[1]
-                and self.u_range[0] <= u <= self.u_range[1])
+                and self.u_range[0] <= u <= self.u_range[1]
+                and (n is None or self.n_range[0] <= n <= self.n_range[1]))
 
-    def predict(self, re, u, extrapolate=False):
+    def predict(self, re, u, extrapolate=False, n=None):
         """(m, m, 3) FP32 device tensor: ((rho - 1)/U^2, ux/U, uy/U) on cell centres.
 
         Raises ``OutOfRange`` (a ``WarmStartUnavailable``, so the controller
-        cold-starts) for (Re, U) outside the trained ranges, unless
-        ``extrapolate``: the inputs are normalized against those ranges, and
-        what the network answers outside them was never fitted."""
-        if not extrapolate and not self.covers(re, u):
-            raise OutOfRange("Re %.6g, U %.6g outside the trained ranges Re %s, U %s"
-                             % (re, u, self.re_range, self.u_range))
+        cold-starts) for (Re, U) outside the trained ranges, or a lattice size
+        n outside the sizes the network is meant for, unless ``extrapolate``:
+        the inputs are normalized against those ranges, and what the network
+        answers outside them was never fitted."""
+        if not extrapolate and not self.covers(re, u, n):
+            raise OutOfRange("Re %.6g, U %.6g, N %s outside the trained ranges Re %s, U %s, N %s"
+                             % (re, u, n, self.re_range, self.u_range, self.n_range))
         return self.net.forward(self.inputs([(re, u)]))[0].contiguous()
 
 
@@ -106,32 +117,35 @@ class PredictorSet:
         self.u_range = (min(m.u_range[0] for m in self.members),
                         max(m.u_range[1] for m in self.members))
 
-    def member(self, re, u):
+    def member(self, re, u, n=None):
         for m in self.members:
-            if m.covers(re, u):
+            if m.covers(re, u, n):
                 return m
         return None
 
-    def covers(self, re, u):
-        return self.member(re, u) is not None
+    def covers(self, re, u, n=None):
+        return self.member(re, u, n) is not None
 
-    def predict(self, re, u, extrapolate=False):
-        m = self.member(re, u)
+    def predict(self, re, u, extrapolate=False, n=None):
+        m = self.member(re, u, n)
         if m is None:
             if not extrapolate:
-                raise OutOfRange("Re %.6g, U %.6g outside every member's ranges" % (re, u))
+                raise OutOfRange("Re %.6g, U %.6g, N %s outside every member's ranges" % (re, u, n))
             m = min(self.members, key=lambda p: abs(math.log(re) - math.log(
                 min(max(re, p.re_range[0]), p.re_range[1]))))
         return m.predict(re, u, extrapolate=extrapolate)
 
 
 def load_default(device=None):
-    """The shipped networks: a ``PredictorSet`` of the low- and high-Reynolds
-    networks when both are present, else the single low-Reynolds ``Predictor``."""
+    """The shipped networks: a ``PredictorSet`` of the networks present (the
+    large-lattice low-Reynolds one first, then the low- and the high-Reynolds
+    ones), or the single low-Reynolds ``Predictor``."""
     low = Predictor.load(DEFAULT_WEIGHTS, device)
-    if not os.path.exists(HIGH_RE_WEIGHTS):
-        return low
-    return PredictorSet([low, Predictor.load(HIGH_RE_WEIGHTS, device)])
+    members = [Predictor.load(LARGE_N_WEIGHTS, device)] if os.path.exists(LARGE_N_WEIGHTS) else []
+    members.append(low)
+    if os.path.exists(HIGH_RE_WEIGHTS):
+        members.append(Predictor.load(HIGH_RE_WEIGHTS, device))
+    return low if len(members) == 1 else PredictorSet(members)
 
 
 RECONSTRUCTIONS = ("bilinear", "walls", "cubic")
@@ -192,7 +206,7 @@ class CavityStepper:
     def init_warm(self):
         if self.predictor is None:
             raise RuntimeError("a warm start needs a predictor")
-        q = self.predictor.predict(self.re, self.u)
+        q = self.predictor.predict(self.re, self.u, n=self.n)
         # the prediction holds averages over cells of n/m x n/m nodes
         m = q.shape[0]
         pool = self.n // m if self.n % m == 0 else getattr(self.predictor, "pool", 2)

@@ -26,7 +26,7 @@ import torch
 
 from .features import RE_RANGE, U_RANGE, in_channels
 from .network import Surrogate
-from .solve import Predictor
+from .solve import ANY_N, Predictor
 
 
 def load_dataset(paths, mean_free_density=True):
@@ -115,11 +115,12 @@ def large_scale_error(pred, X, Y, batch, block=32):
 
 def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, widths=(32, 64, 128),
           mid=128, time_limit=None, log=None, octaves=0, init=None, scales=(), ema=0.0,
-          re_range=RE_RANGE, u_range=U_RANGE):
+          re_range=RE_RANGE, u_range=U_RANGE, n_range=None):
     """Train from scratch, or continue from the predictor saved at ``init``
     (same architecture and data split; a network trained for other input
     ranges is a starting point only).  ``re_range``/``u_range`` normalize the
-    inputs and are the ranges the saved predictor accepts.
+    inputs and are the ranges the saved predictor accepts; ``n_range`` (lo,
+    hi) limits the lattice sizes it is used for (default: any).
 
     With ``ema`` > 0 the saved network is the exponential moving average of
     the weights (decay ``ema`` per step).  At every log point (10 epochs) the
@@ -133,7 +134,8 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
         raise ValueError("training data spans Re %.4g..%.4g, U %.4g..%.4g: outside the ranges %s, %s"
                          % (lo_re, hi_re, lo_u, hi_u, re_range, u_range))
     pred = Predictor(Surrogate(in_ch=in_channels(octaves), widths=widths, mid=mid, seed=seed),
-                     re_range=re_range, u_range=u_range, octaves=octaves)
+                     re_range=re_range, u_range=u_range, octaves=octaves,
+                     n_range=n_range if n_range else ANY_N)
     net = pred.net
     X = pred.inputs(params)
     rng = np.random.default_rng(seed)
@@ -160,7 +162,7 @@ def train(paths, epochs=300, batch=16, lr=2e-3, seed=0, val_frac=0.1, out=None, 
             avg["flat"] = net.ema.cpu().numpy()
             state = folded_state(Surrogate.from_state(avg, device=net.device), mean, std)
         return Predictor.from_state(state, pred.grid, pred.re_range, pred.u_range,
-                                    device=net.device, octaves=octaves)
+                                    device=net.device, octaves=octaves, n_range=pred.n_range)
 
     steps_per_epoch = max(1, len(tr) // batch)
     total = epochs * steps_per_epoch
@@ -226,12 +228,15 @@ def main(argv=None):
     ap.add_argument("--time-limit", type=float, default=None)
     ap.add_argument("--re-range", type=float, nargs=2, default=RE_RANGE)
     ap.add_argument("--u-range", type=float, nargs=2, default=U_RANGE)
+    ap.add_argument("--n-range", type=int, nargs=2, default=None,
+                    help="lattice sizes the saved network is meant for (default: any)")
     ap.add_argument("--out", required=True)
     a = ap.parse_args(argv)
     train(a.data, a.epochs, a.batch, a.lr, a.seed, a.val_frac, a.out, time_limit=a.time_limit,
           log=lambda s: print(s, flush=True), octaves=a.octaves, init=a.init,
           scales=[(int(v.split(":")[0]), float(v.split(":")[1])) for v in a.scale_loss], ema=a.ema,
-          re_range=tuple(a.re_range), u_range=tuple(a.u_range))
+          re_range=tuple(a.re_range), u_range=tuple(a.u_range),
+          n_range=tuple(a.n_range) if a.n_range else None)
 
 
 if __name__ == "__main__":

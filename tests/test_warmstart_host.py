@@ -96,6 +96,33 @@ def test_predictor_refuses_inputs_outside_its_ranges():
     assert issubclass(OutOfRange, WarmStartUnavailable)
 
 
+def test_predictor_set_routes_by_ranges_and_lattice_size():
+    from latticeboltzmann_amd.warmstart.solve import OutOfRange, Predictor, PredictorSet
+
+    class Net:
+        in_ch = IN_CHANNELS
+
+        def __init__(self, tag):
+            self.tag = tag
+
+        def forward(self, x):
+            return [self.tag]
+
+    large = Predictor(Net("large"), re_range=(100.0, 1000.0), u_range=(0.05, 0.1), n_range=(384, 4096))
+    low = Predictor(Net("low"), re_range=(100.0, 1000.0), u_range=(0.05, 0.1))
+    high = Predictor(Net("high"), re_range=(1000.0, 5000.0), u_range=(0.05, 0.1))
+    ps = PredictorSet([large, low, high])
+    assert ps.member(500.0, 0.07, n=512) is large and ps.member(500.0, 0.07, n=256) is low
+    assert ps.member(500.0, 0.07) is large  # no size given: the first member covering (Re, U)
+    assert ps.member(3000.0, 0.07, n=1024) is high and ps.member(1000.0, 0.07, n=256) is low
+    assert not large.covers(500.0, 0.07, n=8192) and not large.covers(500.0, 0.07, n=256)
+    with pytest.raises(OutOfRange):
+        large.predict(500.0, 0.07, n=256)
+    with pytest.raises(OutOfRange):
+        ps.predict(6000.0, 0.07, n=512)
+    assert ps.re_range == (100.0, 5000.0)
+
+
 def test_nan_falls_back_immediately():
     st = MockStepper(lambda k: float("nan"), geometric(1e-3, 0.5))
     r = Controller(tol=1e-6).run(st, warm=True)
