@@ -34,21 +34,31 @@ def test_shipped_weights_load(gpu):
     assert float(q[:, -1, 1].mean()) > 0.3 and abs(float(q[:, 0, 1].mean())) < 0.1
 
 
-@pytest.mark.parametrize("re,u", [(250.0, 0.09), (640.0, 0.06)])
-def test_trained_warm_start_beats_cold_start(gpu, re, u):
-    # held-out (Re, U): not drawn by the training seeds 0-2
+def test_trained_warm_start_three_times_faster(gpu):
+    # the first seven held-out cavities of the evaluation seed (1000; the
+    # training data used seeds 0-2), N = 256: the median wall time to steady
+    # state at least 3 x below the cold start's (the warm time includes the
+    # network and the initializer), no fallback, and every converged field
+    # within 1e-4 U of the cold start's
     from latticeboltzmann_amd.warmstart.controller import Controller
-    from latticeboltzmann_amd.warmstart.solve import Predictor
-    pred = Predictor.load()
+    from latticeboltzmann_amd.warmstart.data import draw_cavities
+    from latticeboltzmann_amd.warmstart.solve import load_default
+    pred = load_default()
     ctl = Controller(tol=1e-6, every=1000)
-    cold, fc = _run(256, re, u, None, ctl)
-    warm, fw = _run(256, re, u, pred, ctl)
-    d = _diff(fw, fc, u)
-    print("Re %.0f U %.2f: cold %d steps %.3f s, warm %d steps %.3f s, diff %.1e U"
-          % (re, u, cold["steps"], cold["seconds"], warm["steps"], warm["seconds"], d))
-    assert cold["reason"] == "converged" and not warm["fell_back"]
-    assert warm["steps"] * 2 <= cold["steps"]
-    assert d < 1e-4
+    _run(256, 300.0, 0.08, pred, ctl)  # first-use costs (graph capture, kernels) out of the timing
+    speedups = []
+    for re, u in draw_cavities(7, 1000, (100.0, 1000.0), (0.05, 0.1)):
+        cold, fc = _run(256, re, u, None, ctl)
+        warm, fw = _run(256, re, u, pred, ctl)
+        d = _diff(fw, fc, u)
+        speedups.append(cold["seconds"] / warm["seconds"])
+        print("Re %.0f U %.3f: cold %d steps %.3f s, warm %d steps %.3f s (%.2f x), diff %.1e U"
+              % (re, u, cold["steps"], cold["seconds"], warm["steps"], warm["seconds"],
+                 speedups[-1], d), flush=True)
+        assert cold["reason"] == "converged" and not warm["fell_back"]
+        assert d < 1e-4
+    print("median wall-time speedup %.2f" % float(np.median(speedups)))
+    assert float(np.median(speedups)) >= 3.0
 
 
 def test_untrained_network_gains_nothing(gpu):
