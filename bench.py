@@ -16,9 +16,10 @@ byte count (torch's, and the library's own 16-byte copy, the practical
 ceiling of a single-pass bandwidth-bound kernel); the ghost-layer sliding-lid
 flow at 4098^2; the run again with the reference's velocity dumps; the
 batched sweep of 512 shear-wave instances (seed 0, FP32 storage, 2000 steps,
-one persistent graph-captured launch per step) with its fitted-viscosity
-errors; and the warm-start time to steady state against the cold start on
-held-out 256 x 256 cavities.
+four steps per launch, graph-captured) with its fitted-viscosity errors; and
+the warm-start time to steady state against the cold start on held-out
+256 x 256 cavities and on a sample drawn as the warm-start workload draws
+them (N 256/512/1024, Re 100-5000).
 
 ``roofline``: the dominant kernel is the K-step launch (K steps per launch,
 the earlier steps kept on chip), which moves about 1/K of a single pass's
@@ -341,24 +342,45 @@ def precision_check(n, omega, steps):
     return out
 
 
-def warm_start(count, n=256, seed=2000):
-    """Held-out cavities: cold vs warm-started steps and wall time to steady state."""
+def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s=150.0):
+    """Held-out cavities: cold vs warm-started steps and wall time to steady
+    state.  ``n256``: ``count`` cavities at N = 256 (Re 100-1000, the low-Reynolds
+    network).  ``config_mix``: up to ``mix_count`` cavities drawn as the warm-start
+    workload draws them (N uniform over 256/512/1024, Re log-uniform in
+    [100, 5000], U uniform in [0.05, 0.1]) with the shipped set of networks,
+    stopping once ``mix_budget_s`` seconds have been spent; draws whose cold
+    start diverges are listed and left out of the summary."""
+    import time as _time
     from latticeboltzmann_amd.warmstart.controller import Controller
-    from latticeboltzmann_amd.warmstart.data import draw_cavities
+    from latticeboltzmann_amd.warmstart.data import draw_cavities, draw_config_cavities
     from latticeboltzmann_amd.warmstart.evaluate import evaluate_one, summarize
-    from latticeboltzmann_amd.warmstart.solve import DEFAULT_WEIGHTS, Predictor
+    from latticeboltzmann_amd.warmstart.solve import DEFAULT_WEIGHTS, Predictor, load_default
     if not os.path.exists(DEFAULT_WEIGHTS):
         return None
-    pred = Predictor.load(DEFAULT_WEIGHTS)
     ctl = Controller(tol=1e-6, every=1000)
+
+    def table(recs):
+        return [{"n": r["n"], "re": round(r["re"], 1), "u": round(r["u"], 4),
+                 "cold_steps": r["cold"]["steps"], "cold_reason": r["cold"]["reason"],
+                 "warm_steps": r["warm"]["steps"] if "warm" in r else None,
+                 "cold_s": round(r["cold"]["seconds"], 3),
+                 "warm_s": round(r["warm"]["seconds"], 3) if "warm" in r else None,
+                 "diff_max_over_U": r["warm"]["diff_max"] if "warm" in r else None} for r in recs]
+
+    pred = Predictor.load(DEFAULT_WEIGHTS)
     recs = [evaluate_one(pred, n, re, u, ctl) for re, u in
             draw_cavities(count, seed, pred.re_range, pred.u_range)]
-    out = summarize(recs)["warm"]
-    out["cavities"] = [{"n": r["n"], "re": round(r["re"], 1), "u": round(r["u"], 4),
-                        "cold_steps": r["cold"]["steps"], "warm_steps": r["warm"]["steps"],
-                        "cold_s": round(r["cold"]["seconds"], 3), "warm_s": round(r["warm"]["seconds"], 3),
-                        "diff_max_over_U": r["warm"]["diff_max"]} for r in recs]
-    out["criterion"] = "max|u(t+1000)-u(t)|/U < 1e-6, FP64, N=%d" % n
+    out = {"n256": dict(summarize(recs)["warm"], cavities=table(recs),
+                        criterion="max|u(t+1000)-u(t)|/U < 1e-6, FP64, N=%d" % n)}
+    full = load_default()
+    recs, t0 = [], _time.perf_counter()
+    for nn, re, u in draw_config_cavities(mix_count, mix_seed, re_range=(100.0, 5000.0)):
+        if _time.perf_counter() - t0 > mix_budget_s:
+            break
+        recs.append(evaluate_one(full, nn, re, u, ctl))
+    s = summarize(recs)
+    mix = dict(s.get("warm", {}), cavities=table(recs), cold_not_converged=s["cold_not_converged"])
+    out["config_mix"] = mix
     return out
 
 

@@ -324,3 +324,38 @@ def velocity_errors(ux, uy, ux_ref, uy_ref, u_ref):
     mag = torch.sqrt(dx * dx + dy * dy)
     return {"l2_over_U": float(torch.sqrt(torch.mean(mag * mag)).item()) / u_ref,
             "linf_over_U": float(mag.max().item()) / u_ref}
+
+
+def _bench(argv=None):
+    """Time a deviation-storage cavity:
+    ``python -m latticeboltzmann_amd.precision [--n 8192] [--fmt f16] [--store-every 4] [--steps 96]``
+    (Re 1000, U 0.1; one JSON line with the MLUPS and the launch shape)."""
+    import argparse
+    import json
+    from .lattice import omega_for_cavity
+    ap = argparse.ArgumentParser(description=_bench.__doc__.split("\n")[0])
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--fmt", default="f16", choices=sorted(FORMATS))
+    ap.add_argument("--store-every", type=int, default=4)
+    ap.add_argument("--rounding", default="stochastic", choices=sorted(ROUNDING))
+    ap.add_argument("--steps", type=int, default=96)
+    a = ap.parse_args(argv)
+    om = omega_for_cavity(a.n, 1000.0, 0.1)
+    lat = DeviationLattice(a.n, a.n, om, "cavity", 0.1, fmt=a.fmt, rounding=a.rounding,
+                           store_every=a.store_every, variant="auto")
+    k = lat.multi_k
+    lat.step(2 * k)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    lat.step(a.steps)
+    end.record()
+    torch.cuda.synchronize()
+    sec = start.elapsed_time(end) / 1e3
+    print(json.dumps({"n": a.n, "fmt": a.fmt, "store_every": a.store_every, "steps": a.steps,
+                      "variant": lat.variant, "multi_variant": lat.multi_variant, "steps_per_launch": k,
+                      "mlups": round(a.n * a.n * a.steps / sec / 1e6, 1)}), flush=True)
+
+
+if __name__ == "__main__":
+    _bench()

@@ -71,7 +71,7 @@ def summarize(kernel, trace, pmc, calib_fetch, calib_write, calib_bytes):
            "counters_per_dispatch": {k: round(v, 3) for k, v in c.items()}}
     if "SQ_INSTS_VALU" in c:
         out["valu_wave_insts_per_launch"] = c["SQ_INSTS_VALU"]
-    if "FETCH_SIZE" in c and "WRITE_SIZE" in c:
+    if "FETCH_SIZE" in c and "WRITE_SIZE" in c and calib_fetch and calib_write and calib_bytes:
         kib = calib_bytes / 1024.0
         f_scale = kib / counter_means([calib_fetch], CALIB_KERNEL)["FETCH_SIZE"]
         w_scale = kib / counter_means([calib_write], CALIB_KERNEL)["WRITE_SIZE"]
@@ -89,9 +89,9 @@ def main(argv=None):
     ap.add_argument("--key", required=True, help="entry name under per_variant (e.g. multi18)")
     ap.add_argument("--trace", required=True)
     ap.add_argument("--pmc", nargs="+", required=True)
-    ap.add_argument("--calib-fetch", required=True)
-    ap.add_argument("--calib-write", required=True)
-    ap.add_argument("--calib-bytes", type=float, required=True)
+    ap.add_argument("--calib-fetch", default=None)
+    ap.add_argument("--calib-write", default=None)
+    ap.add_argument("--calib-bytes", type=float, default=None)
     ap.add_argument("--command", default="", help="the profiled command, recorded in the file")
     ap.add_argument("--note", default="")
     ap.add_argument("--out", required=True)

@@ -13,7 +13,10 @@ threshold.  A warm start is abandoned for a cold start when
 
 * the residual is not finite (the prediction drove the lattice unstable);
 * it stalls: the best residual has not dropped below ``min_drop`` times its
-  value for ``patience`` consecutive checks;
+  value for ``patience`` consecutive checks, or for half of the checks made
+  so far when that is longer (on a large, high-Re cavity the residual
+  oscillates while it falls, and new 10 % lows come 60-80 checks apart a
+  few hundred checks into the run, so the window grows with the run);
 * it runs over budget: more than ``budget`` times ``cold_estimate`` steps
   (when an estimate of the cold-start cost is given).
 
@@ -34,10 +37,12 @@ class WarmStartUnavailable(Exception):
 
 class Controller:
     def __init__(self, tol=1e-6, every=1000, patience=50, min_drop=0.9, budget=0.75,
-                 max_steps=10 ** 7):
-        if not (0 < min_drop < 1) or patience < 1 or every < 1:
-            raise ValueError("need 0 < min_drop < 1, patience >= 1, every >= 1")
+                 max_steps=10 ** 7, patience_frac=0.5):
+        if not (0 < min_drop < 1) or patience < 1 or every < 1 or not (0 <= patience_frac < 1):
+            raise ValueError("need 0 < min_drop < 1, patience >= 1, every >= 1, "
+                             "0 <= patience_frac < 1")
         self.tol, self.every, self.patience = float(tol), int(every), int(patience)
+        self.patience_frac = float(patience_frac)
         self.min_drop, self.budget, self.max_steps = float(min_drop), float(budget), int(max_steps)
 
     def _converge(self, stepper, watch, limit):
@@ -53,7 +58,7 @@ class Controller:
                 return "converged", steps, res
             if res < best * self.min_drop or best == math.inf:
                 best, best_at = res, k
-            elif watch and k - best_at >= self.patience:
+            elif watch and k - best_at >= max(self.patience, self.patience_frac * k):
                 return "stalled", steps, res
             k += 1
         return ("over budget" if watch else "max steps"), steps, res

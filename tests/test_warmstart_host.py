@@ -123,6 +123,21 @@ def test_predictor_set_routes_by_ranges_and_lattice_size():
     assert ps.re_range == (100.0, 5000.0)
 
 
+def test_stall_window_grows_with_the_run():
+    # fast decay for 400 checks, then 20 % drops every 80 checks: past check
+    # 400 half of the run (200+ checks) is longer than the 50-check patience,
+    # so this slow tail is not a stall; with the window fixed it is
+    def warm(k):
+        return 1e-2 * 0.99 ** min(k, 400) * 0.8 ** (max(0, k - 400) // 80)
+
+    st = MockStepper(warm, geometric(1e-2, 0.9))
+    r = Controller(tol=1e-6, patience=50).run(st, warm=True)
+    assert not r["fell_back"] and r["reason"] == "converged"
+    st = MockStepper(warm, geometric(1e-2, 0.9))
+    r = Controller(tol=1e-6, patience=50, patience_frac=0.0).run(st, warm=True)
+    assert r["fell_back"] and r["reason"] == "stalled"
+
+
 def test_nan_falls_back_immediately():
     st = MockStepper(lambda k: float("nan"), geometric(1e-3, 0.5))
     r = Controller(tol=1e-6).run(st, warm=True)
