@@ -33,6 +33,7 @@ INST_DTYPE = np.dtype([("offset", "<i8"), ("nx", "<i4"), ("ny", "<i4"), ("omega"
 TILE_DTYPE = np.dtype([("inst", "<i4"), ("first", "<i4")])
 ITEM_DTYPE = np.dtype([("inst", "<i4"), ("strip", "<i4"), ("row0", "<i4"), ("rows", "<i4")])
 SWEEP_ROWS = 64  # rows per work item of the K-step sweep
+SWEEP_MIN_NY = 64  # a wave of the K-step sweep spans 64 columns
 AMP_TILE = 256  # nodes per amplitude tile (the reduction block size)
 SEED_STEP = 0x85EBCA77  # difference of the dither seeds of consecutive steps
 
@@ -159,8 +160,10 @@ class BatchedLattices:
         if self.multi_k not in (1, 2, 4):
             raise ValueError("steps_per_launch must be 1, 2 or 4")
         if self.multi_k > 1:
-            if any(int(p["nx"]) < 2 * self.multi_k for p in self.instances):
-                raise ValueError("instances need nx >= 2 * steps_per_launch")
+            if any(int(p["nx"]) < 2 * self.multi_k or int(p["ny"]) < SWEEP_MIN_NY
+                   for p in self.instances):
+                raise ValueError("steps_per_launch > 1 needs nx >= 2 * steps_per_launch and "
+                                 "ny >= %d in every instance" % SWEEP_MIN_NY)
             items = sweep_items(table, self.multi_k)
             self._items = _to_device_bytes(items, self.device)
             self._nitems = len(items)

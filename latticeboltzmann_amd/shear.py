@@ -90,8 +90,8 @@ def omega_viscosity_sweep(omegas, n=320, steps=1000, amplitude=0.1, fmt="f32"):
     shear wave (u_x = A sin(2 pi y / n)) per omega, ``steps`` steps, nu from
     the log ratio of the final to the initial amplitude (projection on the
     initial sine).  With n a multiple of 64 every omega runs in one batched
-    lattice set (one persistent launch per step, ``fmt`` storage, FP64
-    collide); otherwise one FP64 lattice per omega.  Returns (nu, nu_theory)
+    lattice set (four steps of every instance per launch, ``fmt`` storage,
+    FP64 collide); otherwise one FP64 lattice per omega.  Returns (nu, nu_theory)
     as arrays in the order of ``omegas``."""
     omegas = [float(w) for w in omegas]
     if n % 64 == 0 and steps % 2 == 0:

@@ -5,10 +5,8 @@
   and total-mass drift against the FP64 run and against the initial mass.
 * Shear wave (default 4096 x 4096, omega 1, amplitude 0.01, 50000 steps):
   viscosity fitted from the amplitude's decay, A(t) ~ A0 exp(-nu k^2 t),
-  against (1/omega - 1/2)/3, per format including FP64; the 16-bit formats
-  with the FP32 collide and with an FP64 collide (FP32 storage always uses
-  the FP64 collide here: an FP32 collide stores unrounded FP32 results and
-  shares the bias of the 16-bit FP32 collide).
+  against (1/omega - 1/2)/3, per format including FP64, each storage format
+  with the FP32 collide and with an FP64 collide.
 * The FP16 cavity is also run with the FP64 collide, and FP16/BF16 storing
   every 3 and every 4 steps (intermediate steps unrounded on chip).
 
@@ -109,12 +107,12 @@ def shear_precision(n=4096, omega=1.0, amp=0.01, steps=50000, every=2500, format
     if log:
         log(json.dumps({"shear": "f64", **out["f64"]}))
     for fmt in formats:
-        for compute in (("f64",) if fmt == "f32" else ("f32", "f64")):
+        for compute in ("f32", "f64"):
             bat = BatchedLattices([dict(nx=n, ny=n, omega=omega, amplitude=amp, direction="x",
                                         bc="periodic")], fmt=fmt, compute=compute)
             (times, amps), sec = _timed(lambda: bat.run(steps, sample_every=every, graph=True))
             nu = _fit(times, amps[:, 0], n)
-            key = fmt if compute == "f32" or fmt == "f32" else fmt + "_f64_collide"
+            key = fmt if compute == "f32" else fmt + "_f64_collide"
             out[key] = {"nu": nu, "nu_rel_err": nu / theory - 1, "seconds": sec,
                         "mlups": n * n * steps / sec / 1e6, "compute": compute}
             if log:
