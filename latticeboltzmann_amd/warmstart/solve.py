@@ -206,7 +206,10 @@ class CavityStepper:
     def init_warm(self):
         if self.predictor is None:
             raise RuntimeError("a warm start needs a predictor")
-        q = self.predictor.predict(self.re, self.u, n=self.n)
+        if isinstance(self.predictor, (Predictor, PredictorSet)):
+            q = self.predictor.predict(self.re, self.u, n=self.n)
+        else:  # any object with predict(re, u)
+            q = self.predictor.predict(self.re, self.u)
         # the prediction holds averages over cells of n/m x n/m nodes
         m = q.shape[0]
         pool = self.n // m if self.n % m == 0 else getattr(self.predictor, "pool", 2)
