@@ -24,12 +24,11 @@ from .network import Surrogate
 _HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_WEIGHTS = os.path.join(_HERE, "surrogate_weights.npz")
 # the network for the upper Reynolds range (trained on cavities with N of 512
-# and 1024, where BGK stays stable up to Re 5000); load_default() routes each
-# cavity to the network whose ranges cover it
+# and 1024, where BGK stays stable up to Re 5000); load_default() averages the
+# networks whose ranges cover a cavity
 HIGH_RE_WEIGHTS = os.path.join(_HERE, "surrogate_weights_high_re.npz")
-# the lower Reynolds range again, trained on cavities with N of 512 and 1024
-# (the steady fields of the larger lattices differ from the 256^2 ones at the
-# level of the network's error), for lattices from 384 nodes on
+# the lower Reynolds range again, fine-tuned on cavities with N of 512 and 1024
+# for lattices from 384 nodes on: averaged with the default network there
 LARGE_N_WEIGHTS = os.path.join(_HERE, "surrogate_weights_large_n.npz")
 ANY_N = (1, 1 << 30)
 
@@ -98,15 +97,24 @@ class Predictor:
 
 
 class PredictorSet:
-    """Several trained surrogates, each for its own (Re, U) ranges: a cavity
-    goes to the first member whose ranges cover it, and a cavity no member
-    covers raises ``OutOfRange`` (the controller then cold-starts).  Members
-    share the cell grid."""
+    """Several trained surrogates, each for its own (Re, U, N) ranges.  With
+    ``combine="first"`` a cavity goes to the first member whose ranges cover
+    it; with ``combine="mean"`` (the shipped set) the predictions of every
+    member covering it are averaged: networks trained apart err partly
+    independently, and the average's large-scale error, the part the
+    lattice's slowest modes must remove, is smaller than either's.  A cavity
+    no member covers raises ``OutOfRange`` (the controller then cold-starts).
+    Members share the cell grid."""
 
-    def __init__(self, members):
+    COMBINE = ("first", "mean")
+
+    def __init__(self, members, combine="first"):
         self.members = list(members)
         if not self.members:
             raise ValueError("a predictor set needs at least one member")
+        if combine not in self.COMBINE:
+            raise ValueError("combine must be one of %s" % (self.COMBINE,))
+        self.combine = combine
         grids = {m.grid for m in self.members}
         if len(grids) != 1:
             raise ValueError("members must share one cell grid, got %s" % sorted(grids))
@@ -117,35 +125,47 @@ class PredictorSet:
         self.u_range = (min(m.u_range[0] for m in self.members),
                         max(m.u_range[1] for m in self.members))
 
+    def covering(self, re, u, n=None):
+        """The members whose ranges cover (Re, U) and the lattice size n."""
+        return [m for m in self.members if m.covers(re, u, n)]
+
     def member(self, re, u, n=None):
-        for m in self.members:
-            if m.covers(re, u, n):
-                return m
-        return None
+        c = self.covering(re, u, n)
+        return c[0] if c else None
 
     def covers(self, re, u, n=None):
         return self.member(re, u, n) is not None
 
     def predict(self, re, u, extrapolate=False, n=None):
-        m = self.member(re, u, n)
-        if m is None:
+        ms = self.covering(re, u, n)
+        if not ms:
             if not extrapolate:
                 raise OutOfRange("Re %.6g, U %.6g, N %s outside every member's ranges" % (re, u, n))
-            m = min(self.members, key=lambda p: abs(math.log(re) - math.log(
-                min(max(re, p.re_range[0]), p.re_range[1]))))
-        return m.predict(re, u, extrapolate=extrapolate)
+            ms = [min(self.members, key=lambda p: abs(math.log(re) - math.log(
+                min(max(re, p.re_range[0]), p.re_range[1]))))]
+        if self.combine == "first" or len(ms) == 1:
+            return ms[0].predict(re, u, extrapolate=extrapolate)
+        q = ms[0].predict(re, u, extrapolate=extrapolate).clone()
+        for m in ms[1:]:
+            q += m.predict(re, u, extrapolate=extrapolate)
+        return q / len(ms)
+
+
+def shipped_weights():
+    """The network files shipped with the package: the low-Reynolds default
+    first, then every other ``surrogate_weights*.npz`` beside it, by name."""
+    others = sorted(f for f in os.listdir(_HERE)
+                    if f.startswith("surrogate_weights") and f.endswith(".npz")
+                    and os.path.join(_HERE, f) != DEFAULT_WEIGHTS)
+    return [DEFAULT_WEIGHTS] + [os.path.join(_HERE, f) for f in others]
 
 
 def load_default(device=None):
-    """The shipped networks: a ``PredictorSet`` of the networks present (the
-    large-lattice low-Reynolds one first, then the low- and the high-Reynolds
-    ones), or the single low-Reynolds ``Predictor``."""
-    low = Predictor.load(DEFAULT_WEIGHTS, device)
-    members = [Predictor.load(LARGE_N_WEIGHTS, device)] if os.path.exists(LARGE_N_WEIGHTS) else []
-    members.append(low)
-    if os.path.exists(HIGH_RE_WEIGHTS):
-        members.append(Predictor.load(HIGH_RE_WEIGHTS, device))
-    return low if len(members) == 1 else PredictorSet(members)
+    """The shipped networks as one ``PredictorSet`` averaging the members that
+    cover a cavity (``combine="mean"``), or the single low-Reynolds
+    ``Predictor`` when it is the only one."""
+    members = [Predictor.load(p, device) for p in shipped_weights()]
+    return members[0] if len(members) == 1 else PredictorSet(members, combine="mean")
 
 
 RECONSTRUCTIONS = ("bilinear", "walls", "cubic")

@@ -138,6 +138,32 @@ def test_stall_window_grows_with_the_run():
     assert r["fell_back"] and r["reason"] == "stalled"
 
 
+def test_predictor_set_mean_averages_the_covering_members():
+    import torch
+    from latticeboltzmann_amd.warmstart.solve import Predictor, PredictorSet
+
+    class Net:
+        in_ch = IN_CHANNELS
+        device = torch.device("cpu")
+
+        def __init__(self, value):
+            self.value = value
+
+        def forward(self, x):
+            return torch.full((x.shape[0], 2, 2, 3), self.value)
+
+    a = Predictor(Net(1.0), re_range=(100.0, 1000.0), u_range=(0.05, 0.1), n_range=(384, 4096))
+    b = Predictor(Net(3.0), re_range=(100.0, 1000.0), u_range=(0.05, 0.1))
+    c = Predictor(Net(10.0), re_range=(1000.0, 5000.0), u_range=(0.05, 0.1))
+    ps = PredictorSet([a, b, c], combine="mean")
+    assert float(ps.predict(500.0, 0.07, n=512)[0, 0, 0]) == 2.0   # a and b
+    assert float(ps.predict(500.0, 0.07, n=256)[0, 0, 0]) == 3.0   # b alone
+    assert float(ps.predict(3000.0, 0.07, n=512)[0, 0, 0]) == 10.0  # c alone
+    assert float(PredictorSet([a, b, c]).predict(500.0, 0.07, n=512)[0, 0, 0]) == 1.0  # first
+    with pytest.raises(ValueError):
+        PredictorSet([a], combine="median")
+
+
 def test_nan_falls_back_immediately():
     st = MockStepper(lambda k: float("nan"), geometric(1e-3, 0.5))
     r = Controller(tol=1e-6).run(st, warm=True)
