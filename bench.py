@@ -344,10 +344,10 @@ def precision_check(n, omega, steps):
 
 def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s=150.0):
     """Held-out cavities: cold vs warm-started steps and wall time to steady
-    state.  ``n256``: ``count`` cavities at N = 256 (Re 100-1000, the low-Reynolds
-    network).  ``config_mix``: up to ``mix_count`` cavities drawn as the warm-start
-    workload draws them (N uniform over 256/512/1024, Re log-uniform in
-    [100, 5000], U uniform in [0.05, 0.1]) with the shipped set of networks,
+    state, warm starts from the shipped set of networks.  ``n256``: ``count``
+    cavities at N = 256 (Re 100-1000).  ``config_mix``: up to ``mix_count``
+    cavities drawn as the warm-start workload draws them (N uniform over
+    256/512/1024, Re log-uniform in [100, 5000], U uniform in [0.05, 0.1]),
     stopping once ``mix_budget_s`` seconds have been spent; draws whose cold
     start diverges are listed and left out of the summary."""
     import time as _time
@@ -367,12 +367,12 @@ def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s
                  "warm_s": round(r["warm"]["seconds"], 3) if "warm" in r else None,
                  "diff_max_over_U": r["warm"]["diff_max"] if "warm" in r else None} for r in recs]
 
-    pred = Predictor.load(DEFAULT_WEIGHTS)
-    recs = [evaluate_one(pred, n, re, u, ctl) for re, u in
-            draw_cavities(count, seed, pred.re_range, pred.u_range)]
+    low = Predictor.load(DEFAULT_WEIGHTS)
+    full = load_default()
+    recs = [evaluate_one(full, n, re, u, ctl) for re, u in
+            draw_cavities(count, seed, low.re_range, low.u_range)]
     out = {"n256": dict(summarize(recs)["warm"], cavities=table(recs),
                         criterion="max|u(t+1000)-u(t)|/U < 1e-6, FP64, N=%d" % n)}
-    full = load_default()
     recs, t0 = [], _time.perf_counter()
     for nn, re, u in draw_config_cavities(mix_count, mix_seed, re_range=(100.0, 5000.0)):
         if _time.perf_counter() - t0 > mix_budget_s:

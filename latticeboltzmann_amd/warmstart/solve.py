@@ -27,9 +27,9 @@ DEFAULT_WEIGHTS = os.path.join(_HERE, "surrogate_weights.npz")
 # and 1024, where BGK stays stable up to Re 5000); load_default() averages the
 # networks whose ranges cover a cavity
 HIGH_RE_WEIGHTS = os.path.join(_HERE, "surrogate_weights_high_re.npz")
-# the lower Reynolds range again, fine-tuned on cavities with N of 512 and 1024
-# for lattices from 384 nodes on: averaged with the default network there
-LARGE_N_WEIGHTS = os.path.join(_HERE, "surrogate_weights_large_n.npz")
+# a second network for the lower Reynolds range, fine-tuned from the default
+# one on cavities with N of 512 and 1024; load_default() averages the two
+LOW_RE_B_WEIGHTS = os.path.join(_HERE, "surrogate_weights_low_re_b.npz")
 ANY_N = (1, 1 << 30)
 
 
