@@ -128,7 +128,8 @@ __global__ __launch_bounds__(BS) void k_step(const T *__restrict__ src, T *__res
 // for a second, serial round of HBM latency.
 template <typename T, int BC, int TR, int TC, int H, int BS>
 __device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *out, int k0, int l0,
-                                                  int nx, int ny, Layout L, T omega, T u0) {
+                                                  int nx, int ny, Layout L, T omega, T u0,
+                                                  bool edge) {
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
     constexpr int V = N > BS ? 2 : 1;
     for (int n0 = threadIdx.x; n0 < N; n0 += V * BS) {
@@ -154,7 +155,7 @@ __device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *
             if (n >= N) continue;
             const int k = kv[v], l = lv[v];
             if constexpr (BC == LBM_BC_CAVITY)
-                cavity_walls(g[v], src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0);
+                if (edge) cavity_walls(g[v], src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0);
             T rho, ux, uy;
             collide_opt1(g[v], omega, rho, ux, uy);
 #pragma unroll
@@ -169,7 +170,8 @@ __device__ __forceinline__ void phase_from_global(const T *__restrict__ src, T *
 // collides them and writes the smaller ring over the old one.  One buffer
 // instead of two halves the LDS of a block.
 template <typename T, int BC, int TR, int TC, int H, int BS>
-__device__ __forceinline__ void phase_lds(T *buf, int k0, int l0, int nx, int ny, T omega, T u0) {
+__device__ __forceinline__ void phase_lds(T *buf, int k0, int l0, int nx, int ny, T omega, T u0,
+                                          bool edge) {
     constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
     constexpr int V = (N + BS - 1) / BS;
@@ -180,13 +182,16 @@ __device__ __forceinline__ void phase_lds(T *buf, int k0, int l0, int nx, int ny
         const int n = threadIdx.x + v * BS;
         if (n >= N) continue;
         const int r = n / W, c = n - r * W;
-        int k = k0 - H + r, l = l0 - H + c;
-        k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
-        l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
         const int m = (r + 1) * Wi + (c + 1);
         pull_lds(buf, Ni, Wi, m, g[v]);
-        if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+        if constexpr (BC == LBM_BC_CAVITY) {
+            if (edge) {
+                int k = k0 - H + r, l = l0 - H + c;
+                k = k < 0 ? k + nx : (k >= nx ? k - nx : k);
+                l = l < 0 ? l + ny : (l >= ny ? l - ny : l);
+                cavity_walls(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+            }
+        }
     }
     __syncthreads();  // every read of the old ring is done
 #pragma unroll
@@ -205,7 +210,7 @@ template <typename T, int BC, bool MOM, int TR, int TC, int BS>
 __device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst, int k0, int l0,
                                                 int nx, int ny, Layout L, T omega, T u0,
                                                 T *__restrict__ rho_o, T *__restrict__ ux_o,
-                                                T *__restrict__ uy_o) {
+                                                T *__restrict__ uy_o, bool edge) {
     constexpr int Wi = TC + 2, Ni = (TR + 2) * Wi;
     for (int n = threadIdx.x; n < TR * TC; n += BS) {
         const int r = n / TC, c = n - r * TC;
@@ -214,7 +219,8 @@ __device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst
         const int m = (r + 1) * Wi + (c + 1);
         T g[9];
         pull_lds(in, Ni, Wi, m, g);
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
+        if constexpr (BC == LBM_BC_CAVITY)
+            if (edge) cavity_walls(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
         T rho, ux, uy;
         collide_opt1(g, omega, rho, ux, uy);
         T *d = dst + int64_t(k) * L.rs + l;
@@ -241,18 +247,24 @@ __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__re
     if (tile >= ntiles) return;  // whole block: before any barrier
     const int tr = tile / ntc, tc = tile - tr * ntc;
     const int k0 = tr * TR, l0 = tc * TC;
+    // whether the phase-1 ring (the largest) reaches a wall row or column or
+    // wraps across the domain's edge; the later rings lie inside it, so
+    // interior tiles skip the wall code in every phase (a block-uniform branch)
+    const bool edge = k0 - (K - 1) <= 0 || k0 + TR + K - 2 >= nx - 1 || l0 - (K - 1) <= 0 ||
+                      l0 + TC + K - 2 >= ny - 1;
 
-    phase_from_global<T, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0);
+    phase_from_global<T, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0, edge);
     __syncthreads();
     if constexpr (K >= 4) {
-        phase_lds<T, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0);
+        phase_lds<T, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0, edge);
         __syncthreads();
     }
     if constexpr (K >= 3) {
-        phase_lds<T, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0);
+        phase_lds<T, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0, edge);
         __syncthreads();
     }
-    phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o);
+    phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o,
+                                            edge);
 }
 
 // ---------------------------------------------------------------------------
