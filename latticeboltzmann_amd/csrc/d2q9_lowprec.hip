@@ -213,7 +213,8 @@ __device__ __forceinline__ void dev_phase_to_global(const typename Codec<FMT>::r
 template <int FMT, typename CT, int BC, int TR, int TC, int H, int BS>
 __device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::raw *__restrict__ src,
                                                     CT *out, int k0, int l0, int nx, int ny,
-                                                    Layout L, CT omega, CT u0, CT inv_scale) {
+                                                    Layout L, CT omega, CT u0, CT inv_scale,
+                                                    bool edge) {
     using C = Codec<FMT>;
     using R = typename C::raw;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
@@ -241,8 +242,9 @@ __device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::r
 #pragma unroll
             for (int i = 0; i < 9; ++i) g[i] = CT(C::dec(raw[v][i])) * inv_scale;
             if constexpr (BC == LBM_BC_CAVITY)
-                cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny, u0,
-                                          inv_scale);
+                if (edge)
+                    cavity_walls_dev<FMT, CT>(g, src + int64_t(k) * L.rs + l, L.cs, k, l, nx, ny,
+                                              u0, inv_scale);
             CT ux, uy;
             collide_dev<CT>(g, omega, ux, uy);
 #pragma unroll
@@ -253,7 +255,7 @@ __device__ __forceinline__ void dev_phase_global_ct(const typename Codec<FMT>::r
 
 template <typename CT, int BC, int TR, int TC, int H, int BS>
 __device__ __forceinline__ void dev_phase_lds_ct(CT *buf, int k0, int l0, int nx, int ny, CT omega,
-                                                 CT u0) {
+                                                 CT u0, bool edge) {
     constexpr int Wi = TC + 2 * (H + 1), Ni = (TR + 2 * (H + 1)) * Wi;
     constexpr int W = TC + 2 * H, N = (TR + 2 * H) * W;
     constexpr int V = (N + BS - 1) / BS;
@@ -264,11 +266,14 @@ __device__ __forceinline__ void dev_phase_lds_ct(CT *buf, int k0, int l0, int nx
         const int n = threadIdx.x + v * BS;
         if (n >= N) continue;
         const int r = n / W, c = n - r * W;
-        const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
         const int m = (r + 1) * Wi + (c + 1);
         pull_lds(buf, Ni, Wi, m, g[v]);
-        if constexpr (BC == LBM_BC_CAVITY)
-            cavity_walls_ct<CT>(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+        if constexpr (BC == LBM_BC_CAVITY) {
+            if (edge) {
+                const int k = wrap1(k0 - H + r, nx), l = wrap1(l0 - H + c, ny);
+                cavity_walls_ct<CT>(g[v], buf + m, int64_t(Ni), k, l, nx, ny, u0);
+            }
+        }
     }
     __syncthreads();  // every read of the old ring is done
 #pragma unroll
@@ -288,7 +293,7 @@ __device__ __forceinline__ void dev_phase_to_global_ct(const CT *in,
                                                        int k0, int l0, int nx, int ny, Layout L,
                                                        CT omega, CT u0, CT scale, int stochastic,
                                                        uint32_t seed, float *__restrict__ ux_o,
-                                                       float *__restrict__ uy_o) {
+                                                       float *__restrict__ uy_o, bool edge) {
     using R = typename Codec<FMT>::raw;
     constexpr int Wi = TC + 2, Ni = (TR + 2) * Wi;
     for (int n = threadIdx.x; n < TR * TC; n += BS) {
@@ -298,7 +303,8 @@ __device__ __forceinline__ void dev_phase_to_global_ct(const CT *in,
         const int m = (r + 1) * Wi + (c + 1);
         CT g[9];
         pull_lds(in, Ni, Wi, m, g);
-        if constexpr (BC == LBM_BC_CAVITY) cavity_walls_ct<CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
+        if constexpr (BC == LBM_BC_CAVITY)
+            if (edge) cavity_walls_ct<CT>(g, in + m, int64_t(Ni), k, l, nx, ny, u0);
         CT ux, uy;
         collide_dev<CT>(g, omega, ux, uy);
         Dither dither(seed, uint64_t(k) * uint64_t(ny) + uint64_t(l));
@@ -333,20 +339,23 @@ __global__ __launch_bounds__(BS) void k_dev_stepk_once(
     const int tr = tile / ntc, tc = tile - tr * ntc;
     const int k0 = tr * TR, l0 = tc * TC;
     const uint32_t last = seed + uint32_t(K - 1) * seed_stride;
+    // interior tiles (phase-1 ring reaching no wall and not wrapping) skip the walls
+    const bool edge = k0 - (K - 1) <= 0 || k0 + TR + K - 2 >= nx - 1 || l0 - (K - 1) <= 0 ||
+                      l0 + TC + K - 2 >= ny - 1;
 
     dev_phase_global_ct<FMT, CT, BC, TR, TC, K - 1, BS>(src, sa, k0, l0, nx, ny, L, omega, u0,
-                                                        inv_scale);
+                                                        inv_scale, edge);
     __syncthreads();
     if constexpr (K >= 4) {
-        dev_phase_lds_ct<CT, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0);
+        dev_phase_lds_ct<CT, BC, TR, TC, 2, BS>(sa, k0, l0, nx, ny, omega, u0, edge);
         __syncthreads();
     }
     if constexpr (K >= 3) {
-        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0);
+        dev_phase_lds_ct<CT, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0, edge);
         __syncthreads();
     }
     dev_phase_to_global_ct<FMT, CT, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0,
-                                                         scale, stochastic, last, ux_o, uy_o);
+                                                         scale, stochastic, last, ux_o, uy_o, edge);
 }
 
 template <int FMT, typename CT, int BC, bool MOM, int K, int TR, int TC, int BS>

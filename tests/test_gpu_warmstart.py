@@ -32,6 +32,16 @@ def test_shipped_weights_load(gpu):
     assert q.shape == (128, 128, 3) and bool(torch.isfinite(q).all())
     # the lid row drives +x flow; the bottom-wall row is nearly at rest
     assert float(q[:, -1, 1].mean()) > 0.3 and abs(float(q[:, 0, 1].mean())) < 0.1
+    # the shipped set: two networks per Reynolds range, averaged
+    from latticeboltzmann_amd.warmstart.solve import PredictorSet, load_default
+    ps = load_default()
+    assert isinstance(ps, PredictorSet) and ps.combine == "mean" and len(ps.members) == 4
+    assert ps.re_range == (100.0, 5000.0)
+    for re in (400.0, 3000.0):
+        ms = ps.covering(re, 0.08, n=512)
+        assert len(ms) == 2
+        mean = (ms[0].predict(re, 0.08) + ms[1].predict(re, 0.08)) / 2
+        assert torch.allclose(ps.predict(re, 0.08, n=512), mean, rtol=1e-6, atol=1e-6)
 
 
 def test_trained_warm_start_three_times_faster(gpu):
