@@ -11,7 +11,8 @@ timed with HIP events on the stream the kernels are launched on.
 
 Also reported (``extra``): the single-step kernel on the same workload; an
 8192 x 8192 cavity (past the 256 MB last-level cache) in FP64 and with
-FP32/BF16/FP16 deviation storage; a device-to-device copy of the 8192^2 FP64
+FP32/BF16/FP16 deviation storage (BF16 and FP16 in their default form, stored
+every 4 steps, and rounded every step); a device-to-device copy of the 8192^2 FP64
 byte count (torch's, and the library's own 16-byte copy, the practical
 ceiling of a single-pass bandwidth-bound kernel); the ghost-layer sliding-lid
 flow at 4098^2; the run again with the reference's velocity dumps; the
@@ -317,9 +318,9 @@ def ghost_flow(n=4096, steps=200):
 
 
 def precision_check(n, omega, steps):
-    """The headline cavity run again in FP16 storage, rounded every step and
-    stored every 3 and every 4 steps, against FP64: velocity errors over U,
-    mass drift."""
+    """The headline cavity run again in FP16 storage, in its default form
+    (stored every 4 steps, ``f16``), stored every 3 steps and rounded every
+    step, against FP64: velocity errors over U, mass drift."""
     import torch
     from latticeboltzmann_amd.lattice import Lattice
     from latticeboltzmann_amd.precision import DeviationLattice, velocity_errors
@@ -329,7 +330,7 @@ def precision_check(n, omega, steps):
     _, uxr, uyr = ref.moments()
     del ref
     out = {"n": n, "steps": steps}
-    for key, every in (("f16", 1), ("f16_store_every_3", 3), ("f16_store_every_4", 4)):
+    for key, every in (("f16_each_step", 1), ("f16_store_every_3", 3), ("f16", 4)):
         lat = DeviationLattice(n, n, omega, "cavity", 0.1, fmt="f16", variant="auto",
                                store_every=every)
         sec = timed_steps(lat, steps, 0)
@@ -467,20 +468,26 @@ def main():
         big = 8192
         om_big = omega_for_cavity(big, 1000.0, 0.1)
         legs = {}
-        for name in ("f64", "f32", "bf16", "f16"):
-            if name == "f64":
+        # the deviation formats in their default form (BF16 and FP16 stored
+        # every 4 steps) and rounding every step
+        for name, fmt, every in (("f64", None, None), ("f32", "f32", None), ("bf16", "bf16", None),
+                                 ("f16", "f16", None), ("bf16_each_step", "bf16", 1),
+                                 ("f16_each_step", "f16", 1)):
+            if fmt is None:
                 lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant,
                               multi_step=not args.single_step)
             else:
-                lat = DeviationLattice(big, big, om_big, "cavity", 0.1, fmt=name, variant="auto")
+                lat = DeviationLattice(big, big, om_big, "cavity", 0.1, fmt=fmt, variant="auto",
+                                       store_every=every)
             steps = 200
             s = timed_steps(lat, steps, 20)
             m = big * big * steps / s / 1e6
             k = lat.multi_k
             legs[name] = {"mlups": round(m, 1), "steps": steps, "variant": lat.variant,
                           "multi_variant": lat.multi_variant, "steps_per_launch": k,
+                          "store_every": getattr(lat, "store_every", 1),
                           # one read and one write of every population per launch
-                          "compulsory_gbs": round(BYTES_PER_LUP[name] / k * m / 1e3, 1)}
+                          "compulsory_gbs": round(BYTES_PER_LUP[fmt or "f64"] / k * m / 1e3, 1)}
             del lat
             torch.cuda.empty_cache()
         lat = Lattice(big, big, om_big, "cavity", 0.1, torch.float64, variant=args.variant,
@@ -492,8 +499,9 @@ def main():
         legs["f64_single_step"] = {"mlups": round(m, 1), "hbm_gbs_algorithmic": round(144 * m / 1e3, 1),
                                    "frac_of_8tbs": round(144 * m / 1e3 / HBM_PEAK_GBS, 4)}
         # FP16 storage against FP64 with temporal blocking (the fastest FP64
-        # form) and against the plain single-pass FP64 kernel
-        for every in (3, 4):
+        # form) and against the plain single-pass FP64 kernel; the default
+        # FP16 form ("f16") stores every 4 steps, here also every 3
+        for every in (3,):
             lat = DeviationLattice(big, big, om_big, "cavity", 0.1, fmt="f16", variant="auto",
                                    store_every=every)
             s = timed_steps(lat, 48 * every, 4 * every)
@@ -505,6 +513,8 @@ def main():
                          "compulsory_gbs": round(36 / every * me / 1e3, 1)}
             legs[key + "_over_f64_mlups"] = round(me / legs["f64"]["mlups"], 3)
         legs["f16_over_f64_mlups"] = round(legs["f16"]["mlups"] / legs["f64"]["mlups"], 3)
+        legs["f16_each_step_over_f64_mlups"] = round(legs["f16_each_step"]["mlups"]
+                                                     / legs["f64"]["mlups"], 3)
         legs["f16_over_f64_single_step_mlups"] = round(legs["f16"]["mlups"] / m, 3)
         extra["cavity_8192"] = legs
         nbytes = 9 * big * big * 8

@@ -28,8 +28,10 @@ def populations(nx, ny, spread=0.05):
 def lattice(nx, ny, fmt, bc="cavity", omega=1.3, layout="rows", variant=-1,
             rounding="stochastic", seed=0, compute="f32"):
     from latticeboltzmann_amd.precision import DeviationLattice
+    # rounding every step: the per-step storage these tests bound
     return DeviationLattice(nx, ny, omega, bc, 0.1, fmt=fmt, scale=SCALES[fmt], layout=layout,
-                            variant=variant, rounding=rounding, seed=seed, compute=compute)
+                            variant=variant, rounding=rounding, seed=seed, compute=compute,
+                            store_every=1)
 
 
 @pytest.mark.parametrize("fmt", lp.FORMATS)
@@ -100,7 +102,7 @@ def test_one_step_fp64_collide_against_oracle(gpu, fmt, rounding):
         f = populations(*shape)
         for v in range(_native.load().lbm_dev_variant_count()):
             lat = DeviationLattice(*shape, om, "cavity", u0, fmt=fmt, scale=s, variant=v,
-                                   rounding=rounding, compute="f64")
+                                   rounding=rounding, compute="f64", store_every=1)
             lat.load(f)
             stored = lat.d.cpu().numpy().view(np.float32 if fmt == "f32" else np.uint16)
             expect = lp.step_reference(stored, fmt, s, om, "cavity", u0, "rows")
@@ -192,7 +194,7 @@ def test_cavity_velocity_error_against_fp64(gpu):
 def test_shear_wave_viscosity_fp16(gpu):
     n, omega, amp = 256, 1.0, 0.01
     from latticeboltzmann_amd.precision import DeviationLattice
-    lat = DeviationLattice(n, n, omega, "periodic", fmt="f16", scale=1024.0)
+    lat = DeviationLattice(n, n, omega, "periodic", fmt="f16", scale=1024.0, store_every=1)
     lat.load(o.shear_wave_init(n, n, amp, "x"))
     ts, amps = [], []
     for _ in range(11):
