@@ -449,6 +449,8 @@ def main():
         detail["hbm_gbs_from_counters"] = round(traffic / launch_s / 1e9, 1)
         detail["hbm_frac_from_counters"] = round(traffic / launch_s / 1e9 / HBM_PEAK_GBS, 4)
         detail["profiled_launch_us"] = prof.get("kernel_trace_avg_us")
+        if "bench_command_trace" in prof:  # this command's own timed loop under the kernel trace
+            detail["profiled_launch_us_this_command"] = prof["bench_command_trace"]["timed_region_avg_us"]
 
     extra = {"finite_state": ok, "omega": omega, "variant": lat_variant,
              "multi_variant": multi_var, "steps_per_launch": multi_k, "dominant_kernel": dominant,
