@@ -8,7 +8,11 @@ cavity: steps and wall time of both (the warm time includes the network's
 inference and the initializer), whether it fell back, and the converged-field
 difference max and RMS |u_warm - u_cold| / U.  With ``--pooled`` a third run
 starts from the cold result itself averaged onto the surrogate's grid: the
-best any predictor on that grid could do.
+best any predictor on that grid could do.  With ``--tight T`` the cold run
+continues past its stop until the residual is below T (e.g. 1e-8), and the
+cold and warm results are also compared with that field: how far each run's
+stopping point sits from the steady state, which bounds what the warm-cold
+difference can resolve.
 
 With several sizes (``--n 256,512,1024``) each cavity's N is drawn with its
 (Re, U) from the seed, uniformly over the sizes.  Cavities whose cold start
@@ -52,11 +56,21 @@ def field_difference(a, b, U):
     return float(d.max()) / U, float(torch.sqrt((d * d).mean())) / U
 
 
-def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic"):
+def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic", tight=None):
     rec = {"n": n, "re": re, "u": u}
+    ft = None
     with CavityStepper(n, re, u) as st:
         cold = ctl.run(st, warm=False)
         fc = tuple(t.clone() for t in st.fields())
+        if tight is not None and cold["reason"] == "converged":
+            # run on from the cold stop to a much smaller residual: the steady state
+            # (at most four times the cold run's steps more)
+            deep = Controller(tol=tight, every=ctl.every,
+                              max_steps=min(ctl.max_steps, 4 * cold["steps"]))
+            status, extra, res = deep._converge(st, False, deep.max_steps)
+            ft = st.fields()
+            rec["tight"] = {"tol": tight, "reason": status, "extra_steps": extra, "residual": res}
+            rec["tight"]["cold_err_max"] = field_difference(fc, ft, u)[0]
     rec["cold"] = {k: cold[k] for k in ("steps", "seconds", "reason")}
     if cold["reason"] != "converged":
         return rec  # no converged yardstick to judge a warm start against
@@ -73,6 +87,8 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic"):
         rec[name].update(diff_max=dmax, diff_rms=drms,
                          step_speedup=cold["steps"] / r["steps"],
                          time_speedup=cold["seconds"] / r["seconds"])
+        if ft is not None:
+            rec[name]["err_max_vs_tight"] = field_difference(fw, ft, u)[0]
     return rec
 
 
@@ -106,6 +122,17 @@ def summarize(records, legs=("warm",)):
             "diff_max_worst": float(max(r["diff_max"] for r in rs)),
             "frac_diff_below_1e-4": float(np.mean([r["diff_max"] < 1e-4 for r in rs])),
         }
+        tight = [(r["tight"]["cold_err_max"], r[leg]["err_max_vs_tight"]) for r in records
+                 if leg in r and "err_max_vs_tight" in r[leg]]
+        if tight:
+            c, w = np.array(tight).T
+            out[leg]["vs_tight"] = {
+                "cavities": len(tight),
+                "cold_err_max_median": float(np.median(c)), "cold_err_max_worst": float(c.max()),
+                "err_max_median": float(np.median(w)), "err_max_worst": float(w.max()),
+                "frac_err_below_1e-4": float(np.mean(w < 1e-4)),
+                "frac_cold_err_below_1e-4": float(np.mean(c < 1e-4)),
+                "frac_not_farther_than_cold": float(np.mean(w <= c))}
     return out
 
 
@@ -123,6 +150,8 @@ def main(argv=None):
     ap.add_argument("--pooled", action="store_true")
     ap.add_argument("--recon", default="cubic", choices=("bilinear", "walls", "cubic"))
     ap.add_argument("--time-limit", type=float, default=None)
+    ap.add_argument("--tight", type=float, default=None,
+                    help="also run the cold start on to this residual and compare both results with it")
     ap.add_argument("--out", default=None)
     a = ap.parse_args(argv)
     pred = load_default() if a.weights == "default" else Predictor.load(a.weights)
@@ -138,7 +167,8 @@ def main(argv=None):
     for n, re, u in cases:
         if a.time_limit and time.time() - t0 > a.time_limit:
             break
-        rec = evaluate_one(pred, n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled, recon=a.recon)
+        rec = evaluate_one(pred, n, re, u, ctl, neq=not a.no_neq, pooled=a.pooled, recon=a.recon,
+                           tight=a.tight)
         records.append(rec)
         print(json.dumps(rec), flush=True)
     summary = summarize(records, ("warm", "pooled") if a.pooled else ("warm",))

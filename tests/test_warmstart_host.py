@@ -267,3 +267,27 @@ def test_evaluation_summary():
     assert [c["n"] for c in out["cold_not_converged"]] == [1024]
     assert out["by_n"]["warm_256"]["cavities"] == 2 and out["by_n"]["warm_512"]["cavities"] == 1
     assert out["by_n"]["warm_512"]["median_time_speedup"] == 2.0 / 0.55
+
+
+def test_evaluation_summary_against_tightened_steady_state():
+    from latticeboltzmann_amd.warmstart.evaluate import summarize
+    recs = []
+    # (cold error, warm error) against the run continued to a smaller residual
+    for i, (ce, we) in enumerate([(3e-5, 2e-5), (1.5e-4, 0.5e-4), (2e-4, 2.5e-4)]):
+        recs.append({"n": 512, "re": 1000.0, "u": 0.1,
+                     "cold": {"steps": 100, "seconds": 1.0, "reason": "converged"},
+                     "tight": {"tol": 1e-8, "cold_err_max": ce},
+                     "warm": {"steps": 30, "seconds": 0.3, "fell_back": False, "diff_max": ce + we,
+                              "step_speedup": 100 / 30, "time_speedup": 1 / 0.3,
+                              "err_max_vs_tight": we}})
+    t = summarize(recs)["warm"]["vs_tight"]
+    assert t["cavities"] == 3
+    assert t["cold_err_max_median"] == 1.5e-4 and t["cold_err_max_worst"] == 2e-4
+    assert t["err_max_median"] == 0.5e-4 and t["err_max_worst"] == 2.5e-4
+    assert np.isclose(t["frac_err_below_1e-4"], 2 / 3)
+    assert np.isclose(t["frac_cold_err_below_1e-4"], 1 / 3)
+    assert np.isclose(t["frac_not_farther_than_cold"], 2 / 3)
+    # without the tightened run there is no such entry
+    for r in recs:
+        del r["warm"]["err_max_vs_tight"]
+    assert "vs_tight" not in summarize(recs)["warm"]
