@@ -774,7 +774,10 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
     X(16, 4, 10, 32, 512)     \
     X(17, 3, 24, 32, 512)     \
     X(18, 4, 20, 32, 512)     \
-    X(19, 4, 24, 28, 512)
+    X(19, 4, 24, 28, 512)     \
+    X(20, 4, 22, 32, 768)     \
+    X(21, 4, 20, 32, 768)     \
+    X(22, 4, 20, 32, 1024)
 #define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
 constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
 
@@ -797,11 +800,14 @@ constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
 #define LBM_COUNT_SWEEP(id, k, seg, pp, pf) +1
 constexpr int kNumSweepShapes = 0 LBM_SWEEP_SHAPES(LBM_COUNT_SWEEP);
 constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
-// Four steps on 20 x 32 tiles with 512 threads: the fastest FP64 shape measured
-// (100-102 GLUPS at 2048^2, 105-106 at 8192^2; 71.1 KB of LDS, so two blocks
-// per CU).  Its four rings (26x38, 24x36, 22x34, 20x32 nodes) take two passes
-// of the block each, with few idle lanes in the second.
-constexpr int kDefaultMultiShape = 18;
+// Four steps on 22 x 32 tiles with 768 threads: the fastest FP64 shape measured
+// (115 GLUPS at 2048^2, 120 at 3072^2, 124.5 at 4096^2, against 105-113, 113-115
+// and 120 for 20 x 32 tiles of 512 threads in the same runs).  76.6 KB of LDS,
+// so two blocks per CU, 24 waves; its rings (28x38, 26x36, 24x34, 22x32 nodes)
+// take 2, 2, 2 and 1 passes.  Blocks of 12 waves give each SIMD 3 waves of a
+// block; blocks whose wave count is not a multiple of 4 (576, 640 threads)
+// load the SIMDs unevenly and ran at 72-78 GLUPS.
+constexpr int kDefaultMultiShape = 20;
 // From 4096^2 nodes on, the five-step sweep with ping-pong row buffers and
 // balanced segments: 124 GLUPS at 4096^2 and 139 at 8192^2 against 114 and
 // 122 for the tile.  At 2048^2 the two tie (105-106); on smaller grids the
