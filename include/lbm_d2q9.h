@@ -106,11 +106,13 @@ int lbm_step_multi_f32(const float *src, float *dst, int64_t nx, int64_t ny, int
                        float omega, int bc, float u0, float *rho, float *ux, float *uy,
                        int variant, void *stream);
 int lbm_step_multi_variant_count(void);
-int lbm_step_multi_default_variant(void); /* the tile default (below 4096^2 nodes) */
+int lbm_step_multi_default_variant(void); /* the FP64 tile default (below 4096^2 nodes) */
 /* The variant that -1 selects on an nx x ny grid: the default tile below
  * 4096^2 nodes, the five-step register sweep from there (LBM_E_SIZE for an
- * invalid grid).                                                             */
+ * invalid grid); _f32 for lbm_step_multi_f32, whose tile default differs.
+ * No counterpart in the reference (its driver steps one step per call).       */
 int lbm_step_multi_default_for(int64_t nx, int64_t ny);
+int lbm_step_multi_default_for_f32(int64_t nx, int64_t ny);
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny);
 
 /* Copy nbytes (a multiple of 16, both pointers 16-byte aligned, disjoint) with

@@ -73,6 +73,7 @@ SIGNATURES = {
     "lbm_step_multi_variant_count": (_i32, []),
     "lbm_step_multi_default_variant": (_i32, []),
     "lbm_step_multi_default_for": (_i32, [_i64, _i64]),
+    "lbm_step_multi_default_for_f32": (_i32, [_i64, _i64]),
     "lbm_step_multi_info": (_i32, [_i32, ctypes.POINTER(_i32), ctypes.POINTER(_i64),
                                    ctypes.POINTER(_i64)]),
     "lbm_relayout": (_i32, [_vp, _vp, _i64, _i64, _i32, _i32, _i32, _vp]),

@@ -809,14 +809,21 @@ constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
 // load the SIMDs unevenly and ran at 72-78 GLUPS.
 constexpr int kDefaultMultiShape = 20;
 // From 4096^2 nodes on, the five-step sweep with ping-pong row buffers and
-// balanced segments: 124 GLUPS at 4096^2 and 139 at 8192^2 against 114 and
-// 122 for the tile.  At 2048^2 the two tie (105-106); on smaller grids the
-// sweep's waves are too few to hide memory latency (64 against 94 GLUPS at 1024^2).
+// balanced segments: 124 GLUPS at 4096^2 (the 22 x 32 tile ties) and 139 at
+// 8192^2 against about 122 for the tiles.  At 2048^2 the sweep's waves are too
+// few to hide memory latency (105 against 115 GLUPS; 64 against 94 at 1024^2).
 constexpr int kLargeGridMultiShape = kNumTileShapes + 10;
 constexpr int64_t kLargeGridNodes = int64_t(4096) * 4096;
+// FP32 lattices below 4096^2 nodes: 20 x 32 tiles with 512 threads (172-190
+// GLUPS at 2048^2 against 168-174 for the 22 x 32, 768-thread tile).  A block
+// holds half the LDS of the FP64 one, so four 8-wave blocks share a CU (32
+// waves) where two 12-wave blocks give 24.
+constexpr int kDefaultMultiShapeF32 = 18;
 
+template <typename T>
 inline int default_multi_shape(int64_t nx, int64_t ny) {
-    return nx * ny >= kLargeGridNodes ? kLargeGridMultiShape : kDefaultMultiShape;
+    if (nx * ny >= kLargeGridNodes) return kLargeGridMultiShape;
+    return sizeof(T) == sizeof(float) ? kDefaultMultiShapeF32 : kDefaultMultiShape;
 }
 
 template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
@@ -883,7 +890,7 @@ int multi_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega
     if (mom != (ux != nullptr) || mom != (uy != nullptr)) return LBM_E_NULL;
     const int64_t bytes = 9 * nx * ny * int64_t(sizeof(T));
     if (overlaps(src, bytes, dst, bytes)) return LBM_E_ALIAS;
-    const int shape = variant < 0 ? default_multi_shape(nx, ny) : variant;
+    const int shape = variant < 0 ? default_multi_shape<T>(nx, ny) : variant;
     hipStream_t s = as_stream(stream);
     const Layout L = make_layout(layout, nx, ny);
     int rc;
@@ -1053,7 +1060,12 @@ int lbm_step_multi_default_variant(void) { return kDefaultMultiShape; }
 
 int lbm_step_multi_default_for(int64_t nx, int64_t ny) {
     if (!valid_grid(nx, ny)) return LBM_E_SIZE;
-    return default_multi_shape(nx, ny);
+    return default_multi_shape<double>(nx, ny);
+}
+
+int lbm_step_multi_default_for_f32(int64_t nx, int64_t ny) {
+    if (!valid_grid(nx, ny)) return LBM_E_SIZE;
+    return default_multi_shape<float>(nx, ny);
 }
 
 int lbm_step_multi_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {

@@ -134,7 +134,10 @@ class Lattice:
         if variant is None:
             return
         if variant < 0:
-            variant = nat.load().lbm_step_multi_default_for(self.nx, self.ny)
+            lib = nat.load()
+            pick = (lib.lbm_step_multi_default_for_f32 if self.dtype == torch.float32
+                    else lib.lbm_step_multi_default_for)
+            variant = pick(self.nx, self.ny)
             if variant < 0:
                 variant = nat.load().lbm_step_multi_default_variant()
         k, mnx, mny = kernels.step_multi_info(variant)

@@ -131,6 +131,12 @@ def test_multi_step_shape_tables_without_gpu():
     assert 0 <= large < lib.lbm_step_multi_variant_count() and large != small
     assert kernels.step_multi_info(large)[2] == 64  # register sweeps need 64 columns
     assert lib.lbm_step_multi_default_for(0, 5) == -2
+    # FP32 lattices: their own tile default below 4096^2, the same sweep from there
+    small32 = lib.lbm_step_multi_default_for_f32(2048, 2048)
+    assert 0 <= small32 < lib.lbm_step_multi_variant_count() and small32 != large
+    assert kernels.step_multi_info(small32)[0] == 4
+    assert lib.lbm_step_multi_default_for_f32(8192, 8192) == large
+    assert lib.lbm_step_multi_default_for_f32(-1, 5) == -2
     for count, info, default in (
             (lib.lbm_step_multi_variant_count(), kernels.step_multi_info,
              lib.lbm_step_multi_default_variant()),
