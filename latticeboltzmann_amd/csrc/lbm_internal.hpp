@@ -67,11 +67,12 @@ inline int64_t resident_waves(F kernel, std::atomic<int> (&per_cu)[64]) {
 
 // Rows per segment of a balanced sweep.  A launch of W waves runs in rounds of
 // R resident waves (CUs x blocks per CU x 4); with 64-row segments the last
-// round is mostly empty (8192^2, K = 5: 18 816 waves = 4.6 rounds of 4 096).
-// Around the round count m0 of 64-row segments, each m in [m0 - 1, m0 + 1]
-// takes the most segments that fit m rounds, and the m with the least
-// m x (segment rows + lead-in rows) wins (8192^2, K = 5: 111 segments of 74
-// rows, 16 317 waves in 4 rounds).  Segments stay at least 16 rows long.
+// round can be mostly empty (8192^2, K = 5, FP64 at 244 VGPRs, two waves per
+// SIMD: 18 816 waves = 9.2 rounds of 2 048).  Around the round count m0 of
+// 64-row segments, each m in [m0 - 1, m0 + 1] takes the most segments that
+// fit m rounds, and the m with the least m x (segment rows + lead-in rows)
+// wins (there: 125 segments of 66 rows, 18 375 waves in 9 rounds, the grid a
+// kernel trace of the launch shows).  Segments stay at least 16 rows long.
 inline int balanced_seg_rows(int64_t nx, int64_t nstrips, int64_t resident, int lead) {
     const int64_t min_rows = nx < 16 ? nx : 16;
     const int64_t w64 = nstrips * ((nx + 63) / 64);
