@@ -63,6 +63,11 @@ class Controller:
             k += 1
         return ("over budget" if watch else "max steps"), steps, res
 
+    def settle(self, stepper, max_steps=None):
+        """Step an already initialised stepper until its residual is below
+        ``tol``, with no stall test: (status, steps, residual)."""
+        return self._converge(stepper, False, self.max_steps if max_steps is None else int(max_steps))
+
     def run(self, stepper, warm=True, cold_estimate=None):
         """Returns a dict: steps (total), warm_steps, cold_steps, fell_back,
         reason, residual, seconds (wall time including initialization)."""

@@ -65,9 +65,8 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic", tig
         if tight is not None and cold["reason"] == "converged":
             # run on from the cold stop to a much smaller residual: the steady state
             # (at most four times the cold run's steps more)
-            deep = Controller(tol=tight, every=ctl.every,
-                              max_steps=min(ctl.max_steps, 4 * cold["steps"]))
-            status, extra, res = deep._converge(st, False, deep.max_steps)
+            deep = Controller(tol=tight, every=ctl.every)
+            status, extra, res = deep.settle(st, min(ctl.max_steps, 4 * cold["steps"]))
             ft = st.fields()
             rec["tight"] = {"tol": tight, "reason": status, "extra_steps": extra, "residual": res}
             rec["tight"]["cold_err_max"] = field_difference(fc, ft, u)[0]

@@ -291,3 +291,15 @@ def test_evaluation_summary_against_tightened_steady_state():
     for r in recs:
         del r["warm"]["err_max_vs_tight"]
     assert "vs_tight" not in summarize(recs)["warm"]
+
+
+def test_settle_runs_on_without_a_stall_test():
+    # a residual that stops improving for a long time still settles (no stall
+    # test), and the step cap ends a run that never gets there
+    st = MockStepper(None, lambda k: 1e-3 if k < 200 else 1e-3 * 0.5 ** (k - 200))
+    st.init_cold()
+    status, steps, res = Controller(tol=1e-8, patience=10).settle(st)
+    assert status == "converged" and res < 1e-8 and steps == 1000 * 217
+    st.init_cold()
+    status, steps, _ = Controller(tol=1e-8).settle(st, max_steps=50_000)
+    assert status == "max steps" and steps == 50_000
