@@ -3,9 +3,10 @@
 Workload (the line's ``value``): the 2048 x 2048 lid-driven cavity in FP64,
 Re = 1000, U = 0.1 (omega = 1/(3 U N / Re + 1/2) = 0.897343), started from
 rho = 1, u = 0 at f_eq, K steps.  A step is one pull stream + walls + collide
-pass over the whole grid.  Runs of 2 to 4 steps go through one launch of the
-multi-step kernel (temporal blocking: the earlier steps of a tile and its
-halo stay in LDS), which computes the bits of as many single-step launches.  Populations stay
+pass over the whole grid.  Runs of four steps go through one launch of the
+multi-step kernel (temporal blocking on 22 x 32 tiles: the earlier steps of a
+tile and its halo stay in LDS), which computes the bits of as many single-step
+launches.  Populations stay
 in HBM; the timed region is K steps bracketed by device synchronisation,
 timed with HIP events on the stream the kernels are launched on.
 
@@ -350,7 +351,11 @@ def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s
     cavities drawn as the warm-start workload draws them (N uniform over
     256/512/1024, Re log-uniform in [100, 5000], U uniform in [0.05, 0.1]),
     stopping once ``mix_budget_s`` seconds have been spent; draws whose cold
-    start diverges are listed and left out of the summary."""
+    start diverges are listed and left out of the summary.  For the mix the
+    cold run also continues to a residual of 1e-8 (at most 4 x its steps
+    more), and both stopping points are compared with that field
+    (``vs_tight``, per cavity ``*_err_vs_steady_over_U``): how far each
+    run's stop sits from the steady state itself."""
     import time as _time
     from latticeboltzmann_amd.warmstart.controller import Controller
     from latticeboltzmann_amd.warmstart.data import draw_cavities, draw_config_cavities
@@ -366,7 +371,11 @@ def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s
                  "warm_steps": r["warm"]["steps"] if "warm" in r else None,
                  "cold_s": round(r["cold"]["seconds"], 3),
                  "warm_s": round(r["warm"]["seconds"], 3) if "warm" in r else None,
-                 "diff_max_over_U": r["warm"]["diff_max"] if "warm" in r else None} for r in recs]
+                 "diff_max_over_U": r["warm"]["diff_max"] if "warm" in r else None,
+                 **({"cold_err_vs_steady_over_U": r["tight"]["cold_err_max"],
+                     "warm_err_vs_steady_over_U": r["warm"].get("err_max_vs_tight"),
+                     "steady_residual": r["tight"]["residual"]} if "tight" in r else {})}
+                for r in recs]
 
     low = Predictor.load(DEFAULT_WEIGHTS)
     full = load_default()
@@ -378,7 +387,7 @@ def warm_start(count, n=256, seed=2000, mix_count=6, mix_seed=5000, mix_budget_s
     for nn, re, u in draw_config_cavities(mix_count, mix_seed, re_range=(100.0, 5000.0)):
         if _time.perf_counter() - t0 > mix_budget_s:
             break
-        recs.append(evaluate_one(full, nn, re, u, ctl))
+        recs.append(evaluate_one(full, nn, re, u, ctl, tight=1e-8))
     s = summarize(recs)
     mix = dict(s.get("warm", {}), cavities=table(recs), cold_not_converged=s["cold_not_converged"])
     out["config_mix"] = mix

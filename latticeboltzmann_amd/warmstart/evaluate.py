@@ -121,17 +121,22 @@ def summarize(records, legs=("warm",)):
             "diff_max_worst": float(max(r["diff_max"] for r in rs)),
             "frac_diff_below_1e-4": float(np.mean([r["diff_max"] < 1e-4 for r in rs])),
         }
-        tight = [(r["tight"]["cold_err_max"], r[leg]["err_max_vs_tight"]) for r in records
-                 if leg in r and "err_max_vs_tight" in r[leg]]
+        # only cavities whose continued run settled below the tightened residual
+        # have a steady state to compare with
+        with_ref = [r for r in records if leg in r and "err_max_vs_tight" in r[leg]]
+        tight = [(r["tight"]["cold_err_max"], r[leg]["err_max_vs_tight"]) for r in with_ref
+                 if r["tight"]["reason"] == "converged"]
+        if with_ref:
+            out[leg]["vs_tight"] = {"cavities": len(tight),
+                                    "no_steady_state": len(with_ref) - len(tight)}
         if tight:
             c, w = np.array(tight).T
-            out[leg]["vs_tight"] = {
-                "cavities": len(tight),
+            out[leg]["vs_tight"].update({
                 "cold_err_max_median": float(np.median(c)), "cold_err_max_worst": float(c.max()),
                 "err_max_median": float(np.median(w)), "err_max_worst": float(w.max()),
                 "frac_err_below_1e-4": float(np.mean(w < 1e-4)),
                 "frac_cold_err_below_1e-4": float(np.mean(c < 1e-4)),
-                "frac_not_farther_than_cold": float(np.mean(w <= c))}
+                "frac_not_farther_than_cold": float(np.mean(w <= c))})
     return out
 
 

@@ -276,12 +276,15 @@ def test_evaluation_summary_against_tightened_steady_state():
     for i, (ce, we) in enumerate([(3e-5, 2e-5), (1.5e-4, 0.5e-4), (2e-4, 2.5e-4)]):
         recs.append({"n": 512, "re": 1000.0, "u": 0.1,
                      "cold": {"steps": 100, "seconds": 1.0, "reason": "converged"},
-                     "tight": {"tol": 1e-8, "cold_err_max": ce},
+                     "tight": {"tol": 1e-8, "cold_err_max": ce, "reason": "converged"},
                      "warm": {"steps": 30, "seconds": 0.3, "fell_back": False, "diff_max": ce + we,
                               "step_speedup": 100 / 30, "time_speedup": 1 / 0.3,
                               "err_max_vs_tight": we}})
-    t = summarize(recs)["warm"]["vs_tight"]
-    assert t["cavities"] == 3
+    # a cavity whose continued run never settles has no steady state to compare with
+    drift = dict(recs[0], tight={"tol": 1e-8, "cold_err_max": 9.0, "reason": "max steps"})
+    drift["warm"] = dict(recs[0]["warm"], err_max_vs_tight=9.0)
+    t = summarize(recs + [drift])["warm"]["vs_tight"]
+    assert t["cavities"] == 3 and t["no_steady_state"] == 1
     assert t["cold_err_max_median"] == 1.5e-4 and t["cold_err_max_worst"] == 2e-4
     assert t["err_max_median"] == 0.5e-4 and t["err_max_worst"] == 2.5e-4
     assert np.isclose(t["frac_err_below_1e-4"], 2 / 3)
