@@ -205,10 +205,15 @@ def test_shear_wave_viscosity_fp16(gpu):
     assert abs(nu / o.viscosity_theory(omega) - 1) < 1e-2
 
 
+# storing every 6 or 8 steps runs on the register sweep, which takes the FP32
+# collide only (checked below)
+STORE_EVERY_CASES = [(c, k) for c in ("f32", "f64") for k in (2, 3, 4, 6, 8)
+                     if not (c == "f64" and k > 4)]
+
+
 @pytest.mark.parametrize("fmt", lp.FORMATS)
-@pytest.mark.parametrize("compute", ["f32", "f64"])
+@pytest.mark.parametrize("compute,K", STORE_EVERY_CASES)
 @pytest.mark.parametrize("rounding", ["nearest", "stochastic"])
-@pytest.mark.parametrize("K", [2, 3, 4, 6, 8])
 def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
     # storing every K steps: K unrounded steps on chip, then one compensated
     # rounding, so the stored state lies within the one-step rounding bounds of
@@ -217,8 +222,6 @@ def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
     w = o.weights()[:, None, None]
     s = SCALES[fmt]
     om, u0 = float(np.float32(1.3)), float(np.float32(0.1))
-    if compute == "f64" and K > 4:
-        pytest.skip("store every 6 or 8 steps: FP32 collide only (register sweep)")
     shape = (24, 80)
     f = populations(*shape)
     for bc in ("cavity", "periodic"):
@@ -241,3 +244,14 @@ def test_store_every_k_against_fp64_oracle(gpu, fmt, compute, rounding, K):
             tol = (4 if rounding == "nearest" else 6) * big[None] + drift
         err = np.abs(x_got - x_exp)
         assert (err <= tol).all(), (bc, float((err / tol).max()))
+
+
+@pytest.mark.parametrize("K", [6, 8])
+def test_store_every_6_and_8_need_the_fp32_collide(gpu, K):
+    # the register sweeps that store every 6 or 8 steps hold FP32 levels only:
+    # asked for explicitly with the FP64 collide, the lattice refuses
+    from latticeboltzmann_amd.precision import DeviationLattice
+    with pytest.raises(ValueError):
+        DeviationLattice(24, 80, 1.3, "cavity", 0.1, fmt="f16", compute="f64", store_every=K)
+    lat = DeviationLattice(24, 80, 1.3, "cavity", 0.1, fmt="f16", compute="f32", store_every=K)
+    assert lat.multi_k == K and not lat.round_each
