@@ -167,7 +167,24 @@ def test_batched_sweep_512_instances_seed0(gpu):
     assert np.isfinite(rel).sum() >= 500
     assert med < 5e-4     # measured 4.8e-5
     assert p90 < 5e-3     # measured 9.6e-4
-    assert worst < 0.05   # measured 2.5e-2 (fast-decaying instances: few points above 1e-3)
+    assert worst < 0.05   # measured 2.5e-2
+    # the worst instances (wavelength 64, omega near 0.3, nu near 0.9) are the
+    # BGK lattice's own departure from (1/omega - 1/2)/3 at that wavelength,
+    # not storage error: the same instance in FP64, sampled and fitted the same
+    # way, gives the same nu (measured: equal to 1e-8)
+    from latticeboltzmann_amd.shear import ShearWave
+    wl = bat.wavelengths()
+    for i in np.argsort(-np.nan_to_num(rel))[:2]:
+        p = inst[i]
+        sw = ShearWave(p["nx"], p["ny"], p["omega"], amplitude=p["amplitude"],
+                       direction=p["direction"])
+        t64, a64 = sw.run(2000, 50)
+        ok = (a64 > 1e-3 * a64[0]) & (t64 >= t64[1])
+        nu64 = -np.polyfit(t64[ok].astype(np.float64), np.log(a64[ok]), 1)[0] / (2 * np.pi / wl[i]) ** 2
+        print("   instance %d (%dx%d, omega %.3f): nu error %.3e batched FP32, %.3e FP64"
+              % (i, p["nx"], p["ny"], p["omega"], nu[i] / theory[i] - 1, nu64 / theory[i] - 1), flush=True)
+        assert abs(nu[i] / nu64 - 1) < 1e-6
+        del sw
     # every instance stays finite and keeps its mass (periodic: no walls)
     for i in (0, 100, 511):
         f = bat.populations(i)
