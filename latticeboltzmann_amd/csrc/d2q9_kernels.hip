@@ -206,7 +206,7 @@ __device__ __forceinline__ void phase_lds(T *buf, int k0, int l0, int nx, int ny
 }
 
 // Last phase: the tile from halo-1 LDS values to dst.
-template <typename T, int BC, bool MOM, int TR, int TC, int BS>
+template <typename T, int BC, bool MOM, int TR, int TC, int BS, bool NTS>
 __device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst, int k0, int l0,
                                                 int nx, int ny, Layout L, T omega, T u0,
                                                 T *__restrict__ rho_o, T *__restrict__ ux_o,
@@ -225,7 +225,7 @@ __device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst
         collide_opt1(g, omega, rho, ux, uy);
         T *d = dst + int64_t(k) * L.rs + l;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) d[i * L.cs] = g[i];
+        for (int i = 0; i < 9; ++i) st<NTS>(d + i * L.cs, g[i]);
         if constexpr (MOM) {
             const int64_t j = int64_t(k) * ny + l;
             rho_o[j] = rho;
@@ -235,7 +235,7 @@ __device__ __forceinline__ void phase_to_global(const T *in, T *__restrict__ dst
     }
 }
 
-template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
+template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS, bool NTS>
 __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__restrict__ dst,
                                                   int nx, int ny, Layout L, int ntc, int ntiles,
                                                   T omega, T u0, T *__restrict__ rho_o,
@@ -263,7 +263,7 @@ __global__ __launch_bounds__(BS) void k_stepk(const T *__restrict__ src, T *__re
         phase_lds<T, BC, TR, TC, 1, BS>(sa, k0, l0, nx, ny, omega, u0, edge);
         __syncthreads();
     }
-    phase_to_global<T, BC, MOM, TR, TC, BS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o,
+    phase_to_global<T, BC, MOM, TR, TC, BS, NTS>(sa, dst, k0, l0, nx, ny, L, omega, u0, rho_o, ux_o, uy_o,
                                             edge);
 }
 
@@ -369,7 +369,7 @@ struct SweepState {
     T P[2][9];
 };
 
-template <typename T, int BC, bool MOM, int K, bool PP, bool PF, int PH>
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF, bool NTS, int PH>
 __device__ __forceinline__ void sweep_iteration(const T *__restrict__ src, T *__restrict__ dst,
                                                 int i, int ks, int ke, int nx, int ny, Layout L,
                                                 const SweepLane<T> &ln, T omega, T u0,
@@ -433,7 +433,7 @@ __device__ __forceinline__ void sweep_iteration(const T *__restrict__ src, T *__
             if (ln.writer && i >= ks) {
                 T *d = dst + int64_t(k) * L.rs + ln.l;
 #pragma unroll
-                for (int q = 0; q < 9; ++q) d[q * L.cs] = h[q];
+                for (int q = 0; q < 9; ++q) lbm::st<NTS>(d + q * L.cs, h[q]);
                 if constexpr (MOM) {
                     const int64_t m = int64_t(k) * ny + ln.l;
                     rho_o[m] = rho;
@@ -452,7 +452,7 @@ __device__ __forceinline__ void sweep_iteration(const T *__restrict__ src, T *__
 // wave w takes segment w / nstrips (rows [seg * seg_rows, + seg_rows)) and
 // strip w % nstrips (columns [strip * (64 - 2(K-1)), + 64 - 2(K-1))).
 // Requires ny >= 64 and nx >= K.
-template <typename T, int BC, bool MOM, int K, bool PP, bool PF>
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF, bool NTS>
 __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ src, T *__restrict__ dst,
                                                int nx, int ny, Layout L, int nstrips, int seg_rows,
                                                int nwaves, int nblocks, T omega, T u0,
@@ -491,13 +491,13 @@ __global__ __launch_bounds__(256) void k_sweep(const T *__restrict__ src, T *__r
     int i = ks - 2 * (K - 1);
     if constexpr (PF) sweep_pull<T, K>(src, i, nx, ny, L, ln, st.P[0]);
     for (; i + 1 < ke; i += 2) {
-        sweep_iteration<T, BC, MOM, K, PP, PF, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
+        sweep_iteration<T, BC, MOM, K, PP, PF, NTS, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
                                                   st, rho_o, ux_o, uy_o);
-        sweep_iteration<T, BC, MOM, K, PP, PF, 1>(src, dst, i + 1, ks, ke, nx, ny, L, ln, omega,
+        sweep_iteration<T, BC, MOM, K, PP, PF, NTS, 1>(src, dst, i + 1, ks, ke, nx, ny, L, ln, omega,
                                                   u0, st, rho_o, ux_o, uy_o);
     }
     if (i < ke)
-        sweep_iteration<T, BC, MOM, K, PP, PF, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
+        sweep_iteration<T, BC, MOM, K, PP, PF, NTS, 0>(src, dst, i, ks, ke, nx, ny, L, ln, omega, u0,
                                                   st, rho_o, ux_o, uy_o);
 }
 
@@ -753,51 +753,54 @@ int step_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega,
 }
 
 // Shapes of the multi-step kernel, indexed by `variant`: (steps per launch K,
-// tile rows TR, tile columns TC, threads per block BS).
-#define LBM_MULTI_SHAPES(X)   \
-    X(0, 2, 8, 64, 256)       \
-    X(1, 2, 4, 128, 256)      \
-    X(2, 2, 16, 32, 256)      \
-    X(3, 2, 4, 64, 256)       \
-    X(4, 3, 8, 32, 256)       \
-    X(5, 3, 4, 64, 256)       \
-    X(6, 4, 8, 32, 256)       \
-    X(7, 2, 8, 64, 512)       \
-    X(8, 2, 8, 128, 512)      \
-    X(9, 3, 8, 32, 512)       \
-    X(10, 3, 12, 32, 512)     \
-    X(11, 3, 8, 64, 512)      \
-    X(12, 4, 8, 32, 512)      \
-    X(13, 3, 16, 32, 512)     \
-    X(14, 4, 12, 32, 512)     \
-    X(15, 4, 16, 32, 512)     \
-    X(16, 4, 10, 32, 512)     \
-    X(17, 3, 24, 32, 512)     \
-    X(18, 4, 20, 32, 512)     \
-    X(19, 4, 24, 28, 512)     \
-    X(20, 4, 22, 32, 768)     \
-    X(21, 4, 20, 32, 768)     \
-    X(22, 4, 20, 32, 1024)
-#define LBM_COUNT_MULTI(id, k, tr, tc, bs) +1
+// tile rows TR, tile columns TC, threads per block BS, non-temporal stores of
+// the finished tile).
+#define LBM_MULTI_SHAPES(X)              \
+    X(0, 2, 8, 64, 256, false)       \
+    X(1, 2, 4, 128, 256, false)      \
+    X(2, 2, 16, 32, 256, false)      \
+    X(3, 2, 4, 64, 256, false)       \
+    X(4, 3, 8, 32, 256, false)       \
+    X(5, 3, 4, 64, 256, false)       \
+    X(6, 4, 8, 32, 256, false)       \
+    X(7, 2, 8, 64, 512, false)       \
+    X(8, 2, 8, 128, 512, false)      \
+    X(9, 3, 8, 32, 512, false)       \
+    X(10, 3, 12, 32, 512, false)     \
+    X(11, 3, 8, 64, 512, false)      \
+    X(12, 4, 8, 32, 512, false)      \
+    X(13, 3, 16, 32, 512, false)     \
+    X(14, 4, 12, 32, 512, false)     \
+    X(15, 4, 16, 32, 512, false)     \
+    X(16, 4, 10, 32, 512, false)     \
+    X(17, 3, 24, 32, 512, false)     \
+    X(18, 4, 20, 32, 512, false)     \
+    X(19, 4, 24, 28, 512, false)     \
+    X(20, 4, 22, 32, 768, false)     \
+    X(21, 4, 20, 32, 768, false)     \
+    X(22, 4, 20, 32, 1024, false)    \
+    X(23, 4, 22, 32, 768, true)
+#define LBM_COUNT_MULTI(id, k, tr, tc, bs, nts) +1
 constexpr int kNumTileShapes = 0 LBM_MULTI_SHAPES(LBM_COUNT_MULTI);
 
 // Shapes of the register sweep (k_sweep), variants kNumTileShapes + id:
 // (steps per launch K, rows per wave segment (0: balanced, balanced_seg_rows),
-// ping-pong buffers, prefetch).
-#define LBM_SWEEP_SHAPES(X)    \
-    X(0, 4, 16, true, true)    \
-    X(1, 4, 32, true, true)    \
-    X(2, 4, 32, false, true)   \
-    X(3, 4, 64, false, false)  \
-    X(4, 5, 32, false, false)  \
-    X(5, 5, 64, true, false)   \
-    X(6, 5, 64, false, false)  \
-    X(7, 5, 64, false, true)   \
-    X(8, 6, 64, false, false)  \
-    X(9, 3, 32, false, true)   \
-    X(10, 5, 0, true, false)   \
-    X(11, 4, 0, true, true)
-#define LBM_COUNT_SWEEP(id, k, seg, pp, pf) +1
+// ping-pong buffers, prefetch, non-temporal stores of the written rows).
+#define LBM_SWEEP_SHAPES(X)              \
+    X(0, 4, 16, true, true, false)    \
+    X(1, 4, 32, true, true, false)    \
+    X(2, 4, 32, false, true, false)   \
+    X(3, 4, 64, false, false, false)  \
+    X(4, 5, 32, false, false, false)  \
+    X(5, 5, 64, true, false, false)   \
+    X(6, 5, 64, false, false, false)  \
+    X(7, 5, 64, false, true, false)   \
+    X(8, 6, 64, false, false, false)  \
+    X(9, 3, 32, false, true, false)   \
+    X(10, 5, 0, true, false, false)   \
+    X(11, 4, 0, true, true, false)    \
+    X(12, 5, 0, true, false, true)
+#define LBM_COUNT_SWEEP(id, k, seg, pp, pf, nts) +1
 constexpr int kNumSweepShapes = 0 LBM_SWEEP_SHAPES(LBM_COUNT_SWEEP);
 constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
 // Four steps on 22 x 32 tiles with 768 threads: the fastest FP64 shape measured
@@ -806,13 +809,17 @@ constexpr int kNumMultiShapes = kNumTileShapes + kNumSweepShapes;
 // so two blocks per CU, 24 waves; its rings (28x38, 26x36, 24x34, 22x32 nodes)
 // take 2, 2, 2 and 1 passes.  Blocks of 12 waves give each SIMD 3 waves of a
 // block; blocks whose wave count is not a multiple of 4 (576, 640 threads)
-// load the SIMDs unevenly and ran at 72-78 GLUPS.
-constexpr int kDefaultMultiShape = 20;
-// From 4096^2 nodes on, the five-step sweep with ping-pong row buffers and
-// balanced segments: 124 GLUPS at 4096^2 (the 22 x 32 tile ties) and 139 at
-// 8192^2 against about 122 for the tiles.  At 2048^2 the sweep's waves are too
-// few to hide memory latency (105 against 115 GLUPS; 64 against 94 at 1024^2).
-constexpr int kLargeGridMultiShape = kNumTileShapes + 10;
+// load the SIMDs unevenly and ran at 72-78 GLUPS.  The finished tile goes to
+// HBM with non-temporal stores (variant 23; 119-120 against 116-117 GLUPS at
+// 2048^2 in the same run): the next launch reads it, this one does not, and
+// the stores then leave L2 to the halo rows neighbouring tiles still read.
+constexpr int kDefaultMultiShape = 23;
+// From 4096^2 nodes on, the five-step sweep with ping-pong row buffers,
+// balanced segments and non-temporal stores: 124-127 GLUPS at 4096^2 (the
+// 22 x 32 tile ties) and 141-144 at 8192^2 (139-141 with plain stores) against
+// about 122 for the tiles.  At 2048^2 the sweep's waves are too few to hide
+// memory latency (105 against 115 GLUPS; 64 against 94 at 1024^2).
+constexpr int kLargeGridMultiShape = kNumTileShapes + 12;
 constexpr int64_t kLargeGridNodes = int64_t(4096) * 4096;
 // FP32 lattices below 4096^2 nodes: 20 x 32 tiles with 512 threads (172-190
 // GLUPS at 2048^2 against 168-174 for the 22 x 32, 768-thread tile).  A block
@@ -826,7 +833,7 @@ inline int default_multi_shape(int64_t nx, int64_t ny) {
     return sizeof(T) == sizeof(float) ? kDefaultMultiShapeF32 : kDefaultMultiShape;
 }
 
-template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS>
+template <typename T, int BC, bool MOM, int K, int TR, int TC, int BS, bool NTS>
 int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
                    T *ux, T *uy, hipStream_t s) {
     if (nx < TR + 2 * K || ny < TC + 2 * K) return LBM_E_SIZE;
@@ -834,13 +841,13 @@ int launch_multi_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T ome
     const int64_t ntiles = ntr * ntc;
     const int64_t nblocks = ((ntiles + 7) / 8) * 8;
     if (nblocks > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL((k_stepk<T, BC, MOM, K, TR, TC, BS>), dim3(unsigned(nblocks)), dim3(BS), 0,
+    hipLaunchKernelGGL((k_stepk<T, BC, MOM, K, TR, TC, BS, NTS>), dim3(unsigned(nblocks)), dim3(BS), 0,
                        s, src, dst, int(nx), int(ny), L, int(ntc), int(ntiles), omega, u0, rho, ux,
                        uy);
     return LBM_OK;
 }
 
-template <typename T, int BC, bool MOM, int K, bool PP, bool PF>
+template <typename T, int BC, bool MOM, int K, bool PP, bool PF, bool NTS>
 int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int seg_rows, T omega,
                    T u0, T *rho, T *ux, T *uy, hipStream_t s) {
     constexpr int VW = 64 - 2 * (K - 1);
@@ -848,7 +855,7 @@ int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int s
     const int64_t nstrips = (ny + VW - 1) / VW;
     if (seg_rows == 0) {
         static std::atomic<int> per_cu[64];
-        seg_rows = balanced_seg_rows(nx, nstrips, resident_waves(k_sweep<T, BC, MOM, K, PP, PF>, per_cu),
+        seg_rows = balanced_seg_rows(nx, nstrips, resident_waves(k_sweep<T, BC, MOM, K, PP, PF, NTS>, per_cu),
                                      2 * (K - 1));
     }
     const int64_t nsegs = (nx + seg_rows - 1) / seg_rows;
@@ -856,7 +863,7 @@ int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int s
     const int64_t nblocks = (nwaves + 3) / 4;
     const int64_t grid = ((nblocks + 7) / 8) * 8;
     if (grid > 0x7fffffffLL) return LBM_E_SIZE;
-    hipLaunchKernelGGL((k_sweep<T, BC, MOM, K, PP, PF>), dim3(unsigned(grid)), dim3(256), 0, s, src, dst,
+    hipLaunchKernelGGL((k_sweep<T, BC, MOM, K, PP, PF, NTS>), dim3(unsigned(grid)), dim3(256), 0, s, src, dst,
                        int(nx), int(ny), L, int(nstrips), seg_rows, int(nwaves), int(nblocks),
                        omega, u0, rho, ux, uy);
     return LBM_OK;
@@ -865,10 +872,10 @@ int launch_sweep_v(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, int s
 template <typename T, int BC, bool MOM>
 int launch_multi_bc(const T *src, T *dst, int64_t nx, int64_t ny, Layout L, T omega, T u0, T *rho,
                     T *ux, T *uy, int shape, hipStream_t s) {
-#define LBM_LAUNCH_MULTI(id, k, tr, tc, bs) \
-    case id: return launch_multi_v<T, BC, MOM, k, tr, tc, bs>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s);
-#define LBM_LAUNCH_SWEEP(id, k, seg, pp, pf) \
-    case kNumTileShapes + id: return launch_sweep_v<T, BC, MOM, k, pp, pf>(src, dst, nx, ny, L, seg, omega, u0, rho, ux, uy, s);
+#define LBM_LAUNCH_MULTI(id, k, tr, tc, bs, nts) \
+    case id: return launch_multi_v<T, BC, MOM, k, tr, tc, bs, nts>(src, dst, nx, ny, L, omega, u0, rho, ux, uy, s);
+#define LBM_LAUNCH_SWEEP(id, k, seg, pp, pf, nts) \
+    case kNumTileShapes + id: return launch_sweep_v<T, BC, MOM, k, pp, pf, nts>(src, dst, nx, ny, L, seg, omega, u0, rho, ux, uy, s);
     switch (shape) {
         LBM_MULTI_SHAPES(LBM_LAUNCH_MULTI)
         LBM_SWEEP_SHAPES(LBM_LAUNCH_SWEEP)
@@ -905,9 +912,9 @@ int multi_impl(const T *src, T *dst, int64_t nx, int64_t ny, int layout, T omega
 
 inline int multi_shape_info(int variant, int *steps, int64_t *min_nx, int64_t *min_ny) {
     const int shape = variant < 0 ? kDefaultMultiShape : variant;
-#define LBM_INFO_MULTI(id, k, tr, tc, bs) \
+#define LBM_INFO_MULTI(id, k, tr, tc, bs, nts) \
     case id: *steps = k; *min_nx = tr + 2 * k; *min_ny = tc + 2 * k; return LBM_OK;
-#define LBM_INFO_SWEEP(id, k, seg, pp, pf) \
+#define LBM_INFO_SWEEP(id, k, seg, pp, pf, nts) \
     case kNumTileShapes + id: *steps = k; *min_nx = 2 * k; *min_ny = 64; return LBM_OK;
     switch (shape) {
         LBM_MULTI_SHAPES(LBM_INFO_MULTI)
