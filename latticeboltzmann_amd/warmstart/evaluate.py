@@ -72,7 +72,7 @@ def evaluate_one(pred, n, re, u, ctl, neq=True, pooled=False, recon="cubic", tig
             rec["tight"]["cold_err_max"] = field_difference(fc, ft, u)[0]
     rec["cold"] = {k: cold[k] for k in ("steps", "seconds", "reason")}
     if cold["reason"] != "converged":
-        return rec  # no converged yardstick to judge a warm start against
+        return rec  # no converged yardstick to measure a warm start against
     legs = [("warm", pred)]
     if pooled:
         m = pred.grid if pred else 128
